@@ -1,0 +1,234 @@
+"""accelerint_amd -- MI355X-native batched Radau IIA-5 behind accelerInt's C API.
+
+Python mirror of the reference library interface
+(generic/solver_interface.cuh:35-55, generic/solver.h:32-33,
+radau2a/radau2a_init.cu:28-31) over ``libaccelerint_hip.so`` via ctypes.
+Arrays follow the reference's layout: ``y`` is column-major, shape
+``(N, NUM)`` in numpy terms (``y[i, tid]`` = ``y_host[i*NUM + tid]``), updated
+in place; ``var`` is the per-condition parameter.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is present
+the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import mechanism  # noqa: F401  (re-export)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaccelerint_hip.so")
+_lib = None
+
+MECH_VAN_DER_POL = 0
+MECH_KINETICS = 2
+NCNT = 10
+CNT_NAMES = ("step", "accept", "reject", "newton_fail", "lu", "lu_fail", "newton", "f", "jac", "es")
+
+# Every symbol include/accelerint.h declares.
+EXPORTS = (
+    "accelerInt_initialize", "accelerInt_integrate", "accelerInt_cleanup", "intDriver",
+    "solver_name", "required_solver_size", "required_mechanism_size", "init_solver_log",
+    "solver_log", "check_error", "accelerInt_select_mechanism", "accelerInt_set_kinetics",
+    "accelerInt_set_tolerances", "accelerInt_set_const_time_step", "accelerInt_set_abort_on_error",
+    "accelerInt_get_status", "accelerInt_get_counters", "accelerInt_reset_counters",
+    "accelerInt_num_states", "accelerInt_integrate_device", "accelerInt_synchronize",
+    "accelerInt_last_error", "accelerInt_eval_callbacks",
+)
+
+
+class AccelerIntError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libaccelerint_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AccelerIntError(
+            "libaccelerint_hip.so is missing at %s; build it with "
+            "`python -m accelerint_amd.build`" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    d, i, v, dp, ip = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    u32p = ctypes.POINTER(ctypes.c_uint32)
+    sig = {
+        "accelerInt_initialize": ([i, i], None),
+        "accelerInt_integrate": ([i, d, d, d, v, v], None),
+        "accelerInt_cleanup": ([], None),
+        "intDriver": ([i, d, d, v, v], None),
+        "solver_name": ([], ctypes.c_char_p),
+        "required_solver_size": ([], ctypes.c_size_t),
+        "required_mechanism_size": ([], ctypes.c_size_t),
+        "init_solver_log": ([], None),
+        "solver_log": ([], None),
+        "check_error": ([i, ip], None),
+        "accelerInt_select_mechanism": ([i], i),
+        "accelerInt_set_kinetics": ([i, i, dp, dp, ip, dp, dp, dp], i),
+        "accelerInt_set_tolerances": ([d, d], i),
+        "accelerInt_set_const_time_step": ([i], i),
+        "accelerInt_set_abort_on_error": ([i], i),
+        "accelerInt_get_status": ([ip, i], i),
+        "accelerInt_get_counters": ([u32p, i], i),
+        "accelerInt_reset_counters": ([], i),
+        "accelerInt_num_states": ([], i),
+        "accelerInt_integrate_device": ([i, d, d, d, v, v, v, v], i),
+        "accelerInt_synchronize": ([], i),
+        "accelerInt_last_error": ([], ctypes.c_char_p),
+        "accelerInt_eval_callbacks": ([i, d, v, v, v, v], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc < 0:
+        raise AccelerIntError("%s failed: %s" % (what, lib().accelerInt_last_error().decode()))
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+# ---- configuration (additive API) ---------------------------------------------
+
+def select_mechanism(kind: int):
+    _check(lib().accelerInt_select_mechanism(kind), "accelerInt_select_mechanism")
+
+
+def set_kinetics(mech) -> None:
+    """Install a synthetic kinetics network (``mechanism.KineticsMechanism``)."""
+    W = np.ascontiguousarray(mech.W, dtype=np.float64)
+    th = np.ascontiguousarray(mech.thermo, dtype=np.float64).reshape(-1)
+    sl = np.ascontiguousarray(mech.slots, dtype=np.int32).reshape(-1)
+    lnA = np.ascontiguousarray(mech.lnA, dtype=np.float64)
+    beta = np.ascontiguousarray(mech.beta, dtype=np.float64)
+    EaR = np.ascontiguousarray(mech.EaR, dtype=np.float64)
+    _check(lib().accelerInt_set_kinetics(int(mech.ns), int(mech.nr), _dp(W), _dp(th), _ip(sl),
+                                         _dp(lnA), _dp(beta), _dp(EaR)), "accelerInt_set_kinetics")
+
+
+def set_tolerances(atol: float, rtol: float):
+    _check(lib().accelerInt_set_tolerances(atol, rtol), "accelerInt_set_tolerances")
+
+
+def set_const_time_step(on: bool):
+    lib().accelerInt_set_const_time_step(int(bool(on)))
+
+
+def set_abort_on_error(on: bool):
+    lib().accelerInt_set_abort_on_error(int(bool(on)))
+
+
+def num_states() -> int:
+    return lib().accelerInt_num_states()
+
+
+# ---- reference interface ------------------------------------------------------
+
+def initialize(NUM: int, device: int = 0):
+    """accelerInt_initialize(int NUM, int device)."""
+    lib().accelerInt_initialize(int(NUM), int(device))
+    err = lib().accelerInt_last_error().decode()
+    if err:
+        raise AccelerIntError(err)
+
+
+def integrate(NUM: int, t_start: float, t_end: float, stepsize: float, y_host: np.ndarray,
+              var_host: np.ndarray):
+    """accelerInt_integrate(NUM, t_start, t_end, stepsize, y_host, var_host).
+
+    ``y_host`` must be a C-contiguous float64 array of N*NUM values in the
+    reference's column-major order (shape ``(N, NUM)``); it is updated in place."""
+    if y_host.dtype != np.float64 or not y_host.flags.c_contiguous:
+        raise TypeError("y_host must be C-contiguous float64")
+    var = np.ascontiguousarray(var_host, dtype=np.float64)
+    if y_host.size != NUM * num_states() or var.size < NUM:
+        raise ValueError("y_host must hold N*NUM values and var_host NUM values")
+    lib().accelerInt_integrate(int(NUM), float(t_start), float(t_end), float(stepsize),
+                               y_host.ctypes.data, var.ctypes.data)
+
+
+def int_driver(NUM: int, t: float, t_end: float, pr_global: np.ndarray, y_global: np.ndarray):
+    """intDriver(NUM, t, t_end, pr_global, y_global): one global step."""
+    if y_global.dtype != np.float64 or not y_global.flags.c_contiguous:
+        raise TypeError("y_global must be C-contiguous float64")
+    pr = np.ascontiguousarray(pr_global, dtype=np.float64)
+    lib().intDriver(int(NUM), float(t), float(t_end), pr.ctypes.data, y_global.ctypes.data)
+
+
+def cleanup():
+    lib().accelerInt_cleanup()
+
+
+def solver_name() -> str:
+    return lib().solver_name().decode()
+
+
+def required_solver_size() -> int:
+    return int(lib().required_solver_size())
+
+
+def required_mechanism_size() -> int:
+    return int(lib().required_mechanism_size())
+
+
+def get_status(num: int) -> np.ndarray:
+    codes = np.zeros(num, dtype=np.int32)
+    rc = lib().accelerInt_get_status(_ip(codes), int(num))
+    if rc < 0:
+        raise AccelerIntError("no integrate call has completed")
+    return codes
+
+
+def get_counters(num: int) -> np.ndarray:
+    out = np.zeros((num, NCNT), dtype=np.uint32)
+    _check(lib().accelerInt_get_counters(out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), int(num)),
+           "accelerInt_get_counters")
+    return out
+
+
+def reset_counters():
+    _check(lib().accelerInt_reset_counters(), "accelerInt_reset_counters")
+
+
+def integrate_device(NUM: int, t_start: float, t_end: float, stepsize: float, y_dev_ptr: int,
+                     var_dev_ptr: int, codes_dev_ptr: int = 0, stream_ptr: int = 0):
+    """Device-resident path: pointers are HBM addresses (e.g. ``tensor.data_ptr()``);
+    y is condition-major (``y[c*N + i]``).  Asynchronous on ``stream_ptr``."""
+    _check(lib().accelerInt_integrate_device(int(NUM), float(t_start), float(t_end), float(stepsize),
+                                             ctypes.c_void_p(y_dev_ptr), ctypes.c_void_p(var_dev_ptr),
+                                             ctypes.c_void_p(codes_dev_ptr or None),
+                                             ctypes.c_void_p(stream_ptr or None)),
+           "accelerInt_integrate_device")
+
+
+def eval_callbacks(NUM: int, t: float, y_host: np.ndarray, var_host: np.ndarray):
+    """Device dydt and Jacobian at each condition's state.  Returns
+    ``dy`` (N, NUM) and ``jac`` (NUM, N, N) with ``jac[c, i, j] = d f_i / d y_j``."""
+    n = num_states()
+    y = np.ascontiguousarray(y_host, dtype=np.float64)
+    var = np.ascontiguousarray(var_host, dtype=np.float64)
+    dy = np.zeros((n, NUM))
+    jac = np.zeros(NUM * n * n)
+    _check(lib().accelerInt_eval_callbacks(int(NUM), float(t), y.ctypes.data, var.ctypes.data,
+                                           dy.ctypes.data, jac.ctypes.data),
+           "accelerInt_eval_callbacks")
+    return dy, jac.reshape(NUM, n, n).transpose(0, 2, 1).copy()
+
+
+def synchronize():
+    _check(lib().accelerInt_synchronize(), "accelerInt_synchronize")

@@ -1,0 +1,586 @@
+// libaccelerint_hip.so: accelerInt-compatible C ABI over the gfx950 batched
+// Radau IIA-5 kernels.
+//
+// Host side mirrors generic/solver_interface.cu:88-225 (initialize / integrate /
+// cleanup), generic/solver_generic.cu:29-42 (one kernel launch per global step)
+// and radau2a/radau2a_init.cu / radau2a_props.cu (solver hooks, check_error).
+// Differences by design: the device layout is condition-major (y[c*N + i]),
+// the transposition to and from the caller's column-major y_host happens on the
+// device, and the per-condition workspace lives in registers/LDS instead of
+// global memory, so required_solver_size() is the few bytes of per-condition
+// HBM state.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "accelerint.h"
+#include "mech_kinetics.hpp"
+#include "mech_vdp.hpp"
+#include "radau_core.hpp"
+
+namespace acc {
+
+constexpr int kBlock = 256;
+
+// One global step [t, t_end] for every condition: the analogue of the
+// reference's intDriver kernel (solver_generic.cu:29-42), one lane group per
+// condition instead of one thread.
+template <class Mech, int G, int R, int BLOCK, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau_step(int num, double t, double t_end,
+                                                       const double* __restrict__ pr,
+                                                       double* __restrict__ y,
+                                                       int* __restrict__ codes,
+                                                       uint32_t* __restrict__ counters,
+                                                       SolverParams P, typename Mech::Params mp) {
+  extern __shared__ double lds_dyn[];
+  constexpr int N = Mech::N;
+  constexpr int GPB = BLOCK / G;
+  const int g = threadIdx.x / G;
+  const int c = blockIdx.x * GPB + g;
+  if (c >= num) return;
+  const int ln = threadIdx.x % G;
+  double yl[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = ln + G * q;
+    yl[q] = (r < N) ? y[(size_t)c * N + r] : 0.0;
+  }
+  typename Mech::Ctx ctx(mp, g, lds_dyn);
+  uint32_t cnt[NCNT];
+#pragma unroll
+  for (int k = 0; k < NCNT; ++k) cnt[k] = 0;
+  const int code = Radau<Mech, G, R>::integrate(ctx, P, t, t_end, pr[c], yl, cnt);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = ln + G * q;
+    if (r < N) y[(size_t)c * N + r] = yl[q];
+  }
+  if (ln == 0) {
+    if (codes) codes[c] = code;
+    if (counters) {
+#pragma unroll
+      for (int k = 0; k < NCNT; ++k) counters[(size_t)c * NCNT + k] += cnt[k];
+    }
+  }
+}
+
+// Evaluates the mechanism callbacks at every condition's state (no
+// integration): dy[c*N + i] and jac[c*N*N + i + j*N] (column-major per
+// condition).  Counterpart of the reference's rate/Jacobian dump test
+// (tests/rateOutputTest.cu:18).
+template <class Mech, int G, int R, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_eval_callbacks(int num, double t,
+                                                          const double* __restrict__ pr,
+                                                          const double* __restrict__ y,
+                                                          double* __restrict__ dy,
+                                                          double* __restrict__ jac,
+                                                          typename Mech::Params mp) {
+  extern __shared__ double lds_dyn[];
+  constexpr int N = Mech::N;
+  constexpr int GPB = BLOCK / G;
+  const int g = threadIdx.x / G;
+  const int c = blockIdx.x * GPB + g;
+  if (c >= num) return;
+  const int ln = threadIdx.x % G;
+  double yl[R], f[R], J[R][N];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = ln + G * q;
+    yl[q] = (r < N) ? y[(size_t)c * N + r] : 0.0;
+  }
+  typename Mech::Ctx ctx(mp, g, lds_dyn);
+  Mech::template dydt<G, R>(ctx, t, pr[c], yl, f);
+  Mech::template jacob<G, R>(ctx, t, pr[c], yl, J);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int r = ln + G * q;
+    if (r < N) {
+      dy[(size_t)c * N + r] = f[q];
+#pragma unroll
+      for (int j = 0; j < N; ++j) jac[(size_t)c * N * N + r + (size_t)j * N] = J[q][j];
+    }
+  }
+}
+
+template <class Mech, int G, int R, int BLOCK>
+hipError_t launch_eval(int num, double t, const double* pr, const double* y, double* dy,
+                       double* jac, const typename Mech::Params& mp, hipStream_t s) {
+  constexpr int GPB = BLOCK / G;
+  const int blocks = (num + GPB - 1) / GPB;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_eval_callbacks<Mech, G, R, BLOCK>), dim3(blocks), dim3(BLOCK),
+                     Mech::lds_bytes(mp, GPB), s, num, t, pr, y, dy, jac, mp);
+  return hipGetLastError();
+}
+
+// Column-major (i*ld + c) <-> condition-major (c*n + i) transposition.
+__global__ void k_cm_to_cond(const double* __restrict__ src, double* __restrict__ dst, int num,
+                             int n, int ld) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)num * n) return;
+  const int c = (int)(idx / n), i = (int)(idx % n);
+  dst[idx] = src[(size_t)i * ld + c];
+}
+__global__ void k_cond_to_cm(const double* __restrict__ src, double* __restrict__ dst, int num,
+                             int n, int ld) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)num * n) return;
+  const int c = (int)(idx % num), i = (int)(idx / num);
+  dst[(size_t)i * ld + c] = src[(size_t)c * n + i];
+}
+
+template <class Mech, int G, int R, int BLOCK, int WPE>
+hipError_t launch_step(int num, double t, double t_end, const double* pr, double* y, int* codes,
+                       uint32_t* cnt, const SolverParams& P, const typename Mech::Params& mp,
+                       hipStream_t s) {
+  constexpr int GPB = BLOCK / G;
+  const int blocks = (num + GPB - 1) / GPB;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_radau_step<Mech, G, R, BLOCK, WPE>), dim3(blocks), dim3(BLOCK),
+                     Mech::lds_bytes(mp, GPB), s, num, t, t_end, pr, y, codes, cnt, P, mp);
+  return hipGetLastError();
+}
+
+}  // namespace acc
+
+using namespace acc;
+
+// ---- library state (one context per process, like solver_interface.cu:17-28) ---
+namespace {
+
+struct State {
+  int mech = ACCELERINT_MECH_VAN_DER_POL;
+  int n = 2;
+  SolverParams P{1e-10, 1e-6, 0};
+  int abort_on_error = 1;
+  int device = -1;
+  int padded = 0;     // capacity in conditions of the device buffers
+  hipStream_t stream = nullptr;
+  double* d_y = nullptr;    // condition-major state
+  double* d_ycm = nullptr;  // column-major staging for host transfers
+  double* d_pr = nullptr;
+  int* d_codes = nullptr;
+  uint32_t* d_cnt = nullptr;
+  std::vector<int> h_codes;
+  int have_status = 0;
+  // kinetics tables (device copies)
+  KineticsHost kin;
+  MechKinetics14::Params kp14{};
+  MechKinetics54::Params kp54{};
+  std::string err;
+} S;
+
+void set_err(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  S.err = buf;
+}
+
+// gpuAssert analogue (examples/van_der_pol/gpu_macros.cuh:26-34): record the
+// error; in abort mode print it and exit with the HIP error code.
+bool hip_ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  set_err("HIP error in %s: %s", what, hipGetErrorString(e));
+  if (S.abort_on_error) {
+    fprintf(stderr, "GPUassert: %s %s\n", hipGetErrorString(e), what);
+    exit((int)e);
+  }
+  return false;
+}
+#define HIPOK(x) hip_ok((x), #x)
+
+size_t bytes_per_condition() {
+  // y (condition-major) + staging copy + parameter + code + counters
+  return (size_t)S.n * sizeof(double) * 2 + sizeof(double) + sizeof(int) + NCNT * sizeof(uint32_t);
+}
+
+void free_buffers() {
+  if (S.d_y) hipFree(S.d_y);
+  if (S.d_ycm) hipFree(S.d_ycm);
+  if (S.d_pr) hipFree(S.d_pr);
+  if (S.d_codes) hipFree(S.d_codes);
+  if (S.d_cnt) hipFree(S.d_cnt);
+  S.d_y = S.d_ycm = S.d_pr = nullptr;
+  S.d_codes = nullptr;
+  S.d_cnt = nullptr;
+  S.padded = 0;
+}
+
+bool alloc_buffers(int cap) {
+  free_buffers();
+  if (!HIPOK(hipMalloc(&S.d_y, (size_t)cap * S.n * sizeof(double)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_ycm, (size_t)cap * S.n * sizeof(double)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_pr, (size_t)cap * sizeof(double)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_codes, (size_t)cap * sizeof(int)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_cnt, (size_t)cap * NCNT * sizeof(uint32_t)))) return false;
+  if (!HIPOK(hipMemset(S.d_cnt, 0, (size_t)cap * NCNT * sizeof(uint32_t)))) return false;
+  S.padded = cap;
+  S.h_codes.assign(cap, 0);
+  return true;
+}
+
+hipError_t launch(int num, double t, double t_end, const double* pr, double* y, int* codes,
+                  uint32_t* cnt, hipStream_t s) {
+  switch (S.mech) {
+    case ACCELERINT_MECH_VAN_DER_POL:
+      return launch_step<MechVdP, 1, 2, 256, 2>(num, t, t_end, pr, y, codes, cnt, S.P, MechVdP::Params{}, s);
+    case ACCELERINT_MECH_KINETICS:
+      if (S.n == 14)
+        return launch_step<MechKinetics14, 16, 1, 128, 2>(num, t, t_end, pr, y, codes, cnt, S.P, S.kp14, s);
+      return hipErrorInvalidValue;
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_eval_any(int num, double t, const double* pr, const double* y, double* dy,
+                           double* jac, hipStream_t s) {
+  switch (S.mech) {
+    case ACCELERINT_MECH_VAN_DER_POL:
+      return launch_eval<MechVdP, 1, 2, 256>(num, t, pr, y, dy, jac, MechVdP::Params{}, s);
+    case ACCELERINT_MECH_KINETICS:
+      if (S.n == 14) return launch_eval<MechKinetics14, 16, 1, 128>(num, t, pr, y, dy, jac, S.kp14, s);
+      return hipErrorInvalidValue;
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+// Global-step loop of solver_interface.cu:225-275 on device-resident,
+// condition-major buffers.
+int run_steps(int num, double t_start, double t_end, double stepsize, double* y, const double* pr,
+              int* codes, uint32_t* cnt, hipStream_t s) {
+  const double step = stepsize < 0 ? t_end - t_start : stepsize;
+  double t = t_start;
+  double t_next = fmin(t_end, t + step);
+  int numSteps = 0;
+  while (t + kRoundoff < t_end) {
+    numSteps++;
+    hipError_t e = launch(num, t, t_next, pr, y, codes, cnt, s);
+    if (!hip_ok(e, "k_radau_step launch")) return -1;
+    t = t_next;
+    t_next = fmin(t_end, t_start + (numSteps + 1) * step);
+  }
+  return 0;
+}
+
+bool ensure_init() {
+  if (S.device >= 0) return true;
+  accelerInt_initialize(0, 0);
+  return S.device >= 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int accelerInt_select_mechanism(int kind) {
+  if (S.device >= 0) {
+    set_err("accelerInt_select_mechanism must precede accelerInt_initialize");
+    return -1;
+  }
+  if (kind == ACCELERINT_MECH_VAN_DER_POL) {
+    S.mech = kind;
+    S.n = 2;
+    return 0;
+  }
+  if (kind == ACCELERINT_MECH_KINETICS) {
+    if (S.kin.ns == 0) {
+      set_err("accelerInt_set_kinetics must be called before selecting the kinetics mechanism");
+      return -1;
+    }
+    S.mech = kind;
+    S.n = S.kin.ns + 1;
+    return 0;
+  }
+  set_err("unknown mechanism kind %d", kind);
+  return -1;
+}
+
+int accelerInt_set_kinetics(int ns, int nr, const double* W, const double* thermo, const int* slots,
+                            const double* lnA, const double* beta, const double* EaR) {
+  if (S.device >= 0) {
+    set_err("accelerInt_set_kinetics must precede accelerInt_initialize");
+    return -1;
+  }
+  if (ns + 1 != 14 && ns + 1 != 54) {
+    set_err("kinetics system size %d is not compiled (available: 14, 54)", ns + 1);
+    return -1;
+  }
+  if (nr <= 0 || nr > kKinMaxReactions) {
+    set_err("kinetics reaction count %d out of range (1..%d)", nr, kKinMaxReactions);
+    return -1;
+  }
+  std::string msg;
+  if (!S.kin.set(ns, nr, W, thermo, slots, lnA, beta, EaR, &msg)) {
+    set_err("%s", msg.c_str());
+    return -1;
+  }
+  S.mech = ACCELERINT_MECH_KINETICS;
+  S.n = ns + 1;
+  return 0;
+}
+
+int accelerInt_set_tolerances(double atol, double rtol) {
+  if (!(atol > 0) || !(rtol > 0)) {
+    set_err("tolerances must be positive");
+    return -1;
+  }
+  S.P.atol = atol;
+  S.P.rtol = rtol;
+  return 0;
+}
+
+int accelerInt_set_const_time_step(int on) {
+  S.P.const_time_step = on ? 1 : 0;
+  return 0;
+}
+
+int accelerInt_set_abort_on_error(int on) {
+  S.abort_on_error = on ? 1 : 0;
+  return 0;
+}
+
+int accelerInt_num_states(void) { return S.n; }
+
+// solver_interface.cu:152-211
+void accelerInt_initialize(int NUM, int device) {
+  device = device < 0 ? 0 : device;
+  int num_devices = 0;
+  if (hipGetDeviceCount(&num_devices) != hipSuccess) num_devices = 0;
+  if (!(device >= 0 && device < num_devices)) {
+    printf("Error: GPU device number not in correct range\n");
+    printf("Provide number between 0 and %i\n", num_devices - 1);
+    exit(1);
+  }
+  if (!HIPOK(hipSetDevice(device))) return;
+  if (S.mech == ACCELERINT_MECH_KINETICS) {
+    std::string msg;
+    if (!S.kin.upload(&msg)) {
+      set_err("%s", msg.c_str());
+      if (S.abort_on_error) {
+        fprintf(stderr, "%s\n", msg.c_str());
+        exit(-1);
+      }
+      return;
+    }
+    S.kp14 = S.kin.params<14>();
+    S.kp54 = S.kin.params<54>();
+  }
+  if (!HIPOK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking))) return;
+  size_t free_mem = 0, total_mem = 0;
+  if (!HIPOK(hipMemGetInfo(&free_mem, &total_mem))) return;
+  const long long max_cond = (long long)floor(0.8 * (double)free_mem / (double)bytes_per_condition());
+  long long cap = NUM > 0 ? NUM : 1;
+  if (cap > max_cond) cap = max_cond;
+  cap = ((cap + kBlock - 1) / kBlock) * kBlock;
+  if (cap <= 0) {
+    printf("Mechanism is too large to fit into global CUDA memory... exiting.");
+    exit(-1);
+  }
+  S.device = device;
+  if (!alloc_buffers((int)cap)) {
+    S.device = -1;
+    return;
+  }
+  S.have_status = 0;
+}
+
+// solver_interface.cu:225-275, with the batches of `padded` conditions kept and
+// the transfers done as one contiguous copy plus a device transposition.
+void accelerInt_integrate(const int NUM, const double t_start, const double t_end,
+                          const double stepsize, double* y_host, const double* var_host) {
+  if (!ensure_init()) return;
+  const int n = S.n;
+  int num_solved = 0;
+  if ((int)S.h_codes.size() < NUM) S.h_codes.assign(NUM, 0);
+  std::vector<int> all_codes(NUM, 0);
+  std::vector<double> stage;
+  while (num_solved < NUM) {
+    const int num_cond = NUM - num_solved < S.padded ? NUM - num_solved : S.padded;
+    if (!HIPOK(hipMemcpyAsync(S.d_pr, var_host + num_solved, num_cond * sizeof(double),
+                              hipMemcpyHostToDevice, S.stream)))
+      return;
+    // rows i of the caller's column-major array, columns [num_solved, +num_cond)
+    if (!HIPOK(hipMemcpy2DAsync(S.d_ycm, (size_t)num_cond * sizeof(double), y_host + num_solved,
+                                (size_t)NUM * sizeof(double), (size_t)num_cond * sizeof(double), n,
+                                hipMemcpyHostToDevice, S.stream)))
+      return;
+    const size_t tot = (size_t)num_cond * n;
+    const int tb = 256;
+    const int nb = (int)((tot + tb - 1) / tb);
+    hipLaunchKernelGGL(k_cm_to_cond, dim3(nb), dim3(tb), 0, S.stream, S.d_ycm, S.d_y, num_cond, n,
+                       num_cond);
+    if (!HIPOK(hipGetLastError())) return;
+    if (run_steps(num_cond, t_start, t_end, stepsize, S.d_y, S.d_pr, S.d_codes,
+                  S.d_cnt + (size_t)0, S.stream) != 0)
+      return;
+    hipLaunchKernelGGL(k_cond_to_cm, dim3(nb), dim3(tb), 0, S.stream, S.d_y, S.d_ycm, num_cond, n,
+                       num_cond);
+    if (!HIPOK(hipGetLastError())) return;
+    if (!HIPOK(hipMemcpyAsync(all_codes.data() + num_solved, S.d_codes, num_cond * sizeof(int),
+                              hipMemcpyDeviceToHost, S.stream)))
+      return;
+    if (!HIPOK(hipMemcpy2DAsync(y_host + num_solved, (size_t)NUM * sizeof(double), S.d_ycm,
+                                (size_t)num_cond * sizeof(double), (size_t)num_cond * sizeof(double),
+                                n, hipMemcpyDeviceToHost, S.stream)))
+      return;
+    if (!HIPOK(hipStreamSynchronize(S.stream))) return;
+    num_solved += num_cond;
+  }
+  S.h_codes.assign(all_codes.begin(), all_codes.end());
+  S.have_status = 1;
+  if (S.abort_on_error) check_error(NUM, all_codes.data());
+}
+
+void intDriver(const int NUM, const double t, const double t_end, const double* pr_global,
+               double* y_global) {
+  if (!ensure_init()) return;
+  accelerInt_integrate(NUM, t, t_end, t_end - t, y_global, pr_global);
+}
+
+int accelerInt_integrate_device(int NUM, double t_start, double t_end, double stepsize,
+                                double* y_dev, const double* var_dev, int* codes_dev,
+                                void* stream) {
+  if (!ensure_init()) return -1;
+  if (NUM < 0 || (NUM > 0 && (!y_dev || !var_dev))) {
+    set_err("accelerInt_integrate_device: bad arguments");
+    return -2;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : S.stream;
+  uint32_t* cnt = (NUM <= S.padded) ? S.d_cnt : nullptr;
+  return run_steps(NUM, t_start, t_end, stepsize, y_dev, var_dev, codes_dev, cnt, s);
+}
+
+int accelerInt_eval_callbacks(int NUM, double t, const double* y_host, const double* var_host,
+                              double* dy_host, double* jac_host) {
+  if (!ensure_init()) return -1;
+  if (NUM <= 0 || NUM > S.padded || !y_host || !var_host || !dy_host || !jac_host) {
+    set_err("accelerInt_eval_callbacks: bad arguments (NUM must be 1..%d)", S.padded);
+    return -2;
+  }
+  const int n = S.n;
+  double *d_dy = nullptr, *d_jac = nullptr;
+  if (!HIPOK(hipMalloc(&d_dy, (size_t)NUM * n * sizeof(double)))) return -1;
+  if (!HIPOK(hipMalloc(&d_jac, (size_t)NUM * n * n * sizeof(double)))) {
+    hipFree(d_dy);
+    return -1;
+  }
+  int rc = 0;
+  const size_t tot = (size_t)NUM * n;
+  const int nb = (int)((tot + 255) / 256);
+  std::vector<double> dy_cm((size_t)NUM * n);
+  if (!HIPOK(hipMemcpyAsync(S.d_pr, var_host, NUM * sizeof(double), hipMemcpyHostToDevice, S.stream)) ||
+      !HIPOK(hipMemcpyAsync(S.d_ycm, y_host, tot * sizeof(double), hipMemcpyHostToDevice, S.stream))) {
+    rc = -1;
+  } else {
+    hipLaunchKernelGGL(k_cm_to_cond, dim3(nb), dim3(256), 0, S.stream, S.d_ycm, S.d_y, NUM, n, NUM);
+    if (!hip_ok(launch_eval_any(NUM, t, S.d_pr, S.d_y, d_dy, d_jac, S.stream), "k_eval_callbacks"))
+      rc = -1;
+    else {
+      hipLaunchKernelGGL(k_cond_to_cm, dim3(nb), dim3(256), 0, S.stream, d_dy, S.d_ycm, NUM, n, NUM);
+      if (!HIPOK(hipMemcpyAsync(dy_host, S.d_ycm, tot * sizeof(double), hipMemcpyDeviceToHost, S.stream)) ||
+          !HIPOK(hipMemcpyAsync(jac_host, d_jac, tot * n * sizeof(double), hipMemcpyDeviceToHost, S.stream)) ||
+          !HIPOK(hipStreamSynchronize(S.stream)))
+        rc = -1;
+    }
+  }
+  hipFree(d_dy);
+  hipFree(d_jac);
+  return rc;
+}
+
+int accelerInt_synchronize(void) {
+  if (S.device < 0) return 0;
+  return HIPOK(hipStreamSynchronize(S.stream)) ? 0 : -1;
+}
+
+void accelerInt_cleanup(void) {
+  if (S.device < 0) return;
+  hipStreamSynchronize(S.stream);
+  free_buffers();
+  S.kin.release();
+  hipStreamDestroy(S.stream);
+  S.stream = nullptr;
+  S.device = -1;
+  S.have_status = 0;
+}
+
+int accelerInt_get_status(int* codes, int num) {
+  if (!S.have_status) return -1;
+  int bad = 0;
+  const int m = num < (int)S.h_codes.size() ? num : (int)S.h_codes.size();
+  for (int i = 0; i < m; ++i) {
+    if (codes) codes[i] = S.h_codes[i];
+    if (S.h_codes[i] != 0) ++bad;
+  }
+  return bad;
+}
+
+int accelerInt_get_counters(uint32_t* out, int num) {
+  if (S.device < 0 || num > S.padded) return -1;
+  if (!HIPOK(hipStreamSynchronize(S.stream))) return -1;
+  return HIPOK(hipMemcpy(out, S.d_cnt, (size_t)num * NCNT * sizeof(uint32_t), hipMemcpyDeviceToHost))
+             ? 0
+             : -1;
+}
+
+int accelerInt_reset_counters(void) {
+  if (S.device < 0) return -1;
+  return HIPOK(hipMemsetAsync(S.d_cnt, 0, (size_t)S.padded * NCNT * sizeof(uint32_t), S.stream))
+             ? 0
+             : -1;
+}
+
+const char* accelerInt_last_error(void) { return S.err.c_str(); }
+
+// radau2a_init.cu:28-31
+const char* solver_name(void) { return "radau2a-int-gpu"; }
+
+size_t required_solver_size(void) {
+  // condition-major state + staging + return code + counters; the Radau
+  // workspace itself (E1, E2, stages, interpolant) is register/LDS resident.
+  return (size_t)S.n * sizeof(double) * 2 + sizeof(int) + NCNT * sizeof(uint32_t);
+}
+
+size_t required_mechanism_size(void) { return sizeof(double); }
+
+void init_solver_log(void) {}
+void solver_log(void) {}
+
+// radau2a_props.cu:15-40
+void check_error(int num_cond, int* codes) {
+  for (int tid = 0; tid < num_cond; ++tid) {
+    const int code = codes[tid];
+    switch (code) {
+      case EC_consecutive_steps:
+        printf("During integration of ODE# %d, an error occured on too many consecutive integration steps,"
+               "exiting...\n", tid);
+        exit(code);
+      case EC_max_steps_exceeded:
+        printf("During integration of ODE# %d, the allowed number of integration steps was exceeded,"
+               "exiting...\n", tid);
+        exit(code);
+      case EC_h_plus_t_equals_h:
+        printf("During integration of ODE# %d, the stepsize 'h' was decreased such that h = t + h,"
+               "exiting...\n", tid);
+        exit(code);
+      case EC_newton_max_iterations_exceeded:
+        printf("During integration of ODE# %d, the allowed number of newton iteration steps was exceeded,"
+               "exiting...\n", tid);
+        exit(code);
+    }
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,209 @@
+"""Benchmark: conditions/s per 1e-6 s global step of the batched Radau IIA-5
+integrator on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+
+Workload (default ``c1``): the 14-state synthetic constant-pressure kinetics
+network (13 species + T, 27 reversible reactions), 262144 conditions with
+T ~ U[1000, 1600] K, P ~ U[1, 25] atm, Y ~ Dirichlet(0.5), plain FP64 Radau,
+rtol 1e-6, atol 1e-10, global steps of 1e-6 s.  A "step" is one global step of
+all conditions (one kernel launch); inputs are resident in HBM before timing.
+
+Prints one JSON line.  ``roofline`` reports the integration kernel's
+algorithmic FP64 FLOP rate (from per-condition event counters) against the
+FP64 peak; ``cpu_baseline`` times the CPU oracle (C restatement of the
+reference's OpenMP radau2a.c path) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+CONFIGS = {
+    # name: (n_states, num, dt, T_range, description)
+    "c0": (2, 4096, 1e-3, None, "van der Pol, mu=10^U(1,3), 4096 conditions, dt=1e-3"),
+    "c1": (14, 262144, 1e-6, (1000.0, 1600.0),
+           "synthetic kinetics 14 states (13 species + T), 27 reactions, 262144 conditions, dt=1e-6 s"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def callback_flops(mech):
+    """Static per-call FLOP counts of the kinetics callbacks (exp/log count 1):
+    dydt: thermo per species (~22), mixture sums (6/species), concentrations
+    (3/species), per reaction 6 slot terms (4 each), rate constants (~12 incl.
+    2 exp), products and q (4); incidence accumulation (2 per entry); T row.
+    Jacobian: dydt work plus dq/dC (per slot pair ~3), Jc row dot (2 NS^2),
+    dense transform (4 NS^2), T row (6 NS)."""
+    ns, nr = mech.ns, mech.nr
+    nnz = 0
+    for j in range(nr):
+        sl = mech.slots[j]
+        nnz += len(set(int(s) for s in sl if s >= 0))
+    F = ns * (22 + 6 + 3) + nr * (6 * 4 + 12 + 4) + 2 * nnz + 3 * ns + 10
+    Jc = F + nr * 6 * 6 * 3 + 2 * ns * ns + 4 * ns * ns + 6 * ns + 20
+    return F, Jc
+
+
+def algorithmic_flops(cnt, n, F, Jc):
+    """Per-event work of the Radau step (same definition for GPU and CPU):
+    LU pair (2/3 N^3 real + 8/3 N^3 complex + 2 N^2 assembly), Newton iteration
+    (10 N^2 solves + 66 N vector work), error-estimate solve 2 N^2 + 8 N,
+    callbacks F / Jc, and ~40 N per step of controller vector work."""
+    c = cnt.sum(0).astype(np.float64)
+    step, acc_, rej, nfail, lu, lufail, newton, f, jac, es = c
+    return (lu * ((10.0 / 3.0) * n ** 3 + 2 * n * n) + newton * (10 * n * n + 66 * n)
+            + es * (2 * n * n + 8 * n) + f * F + jac * Jc + step * 40 * n)
+
+
+def cpu_baseline(cfg, mech, y0, pr, dt, target_cpu_s=12.0):
+    """Times the CPU oracle with all host cores on a bounded sample of the
+    workload (first conditions, first global steps)."""
+    import oracle
+    oracle.build()
+    n = cfg[0]
+    om = oracle.Mechanism.van_der_pol() if n == 2 else oracle.Mechanism.kinetics(mech)
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    cores = min(cores, 16)
+    # probe
+    num = min(1024, y0.shape[1])
+    t0 = time.perf_counter()
+    oracle.integrate_steps(om, y0[:, :num], pr[:num], dt, 2, nthreads=cores)
+    rate = num * 2 / max(time.perf_counter() - t0, 1e-6)
+    steps = 20 if n != 2 else 100
+    num = int(min(y0.shape[1], max(cores * 64, rate * target_cpu_s / cores / steps)))
+    num = max(cores, num - num % cores)
+    t0 = time.perf_counter()
+    oracle.integrate_steps(om, y0[:, :num], pr[:num], dt, steps, nthreads=cores)
+    wall = time.perf_counter() - t0
+    return {"value": num * steps / wall, "unit": "conditions*steps/s", "cores": cores, "kind": "port",
+            "sample": "CPU oracle (C/OpenMP restatement of radau2a.c), first %d conditions x %d global "
+                      "steps of dt=%g, %d OpenMP threads, %.2f s wall" % (num, steps, dt, cores, wall)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--num", type=int, default=0, help="override condition count")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (the integrator shards by condition; "
+              "multi-GPU runs are not part of it); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import accelerint_amd as acc
+    from accelerint_amd import mechanism as M
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    cfg = CONFIGS[args.config]
+    n, num, dt, T_range, desc = cfg
+    if args.num:
+        num = args.num
+    if n == 2:
+        mech = None
+        y0, pr = M.van_der_pol_conditions(num, seed=1234)
+        acc.select_mechanism(acc.MECH_VAN_DER_POL)
+        F = Jc = 8
+    else:
+        mech = M.network(n)
+        y0, pr = M.initial_conditions(mech, num, seed=20261202 + 1, T_range=T_range)
+        acc.set_kinetics(mech)
+        F, Jc = callback_flops(mech)
+    acc.set_tolerances(1e-10, 1e-6)
+    acc.set_abort_on_error(False)
+    acc.initialize(num, 0)
+
+    dev = torch.device("cuda:0")
+    y_dev = torch.from_numpy(np.ascontiguousarray(y0.T)).to(dev)  # condition-major
+    pr_dev = torch.from_numpy(pr).to(dev)
+    codes_dev = torch.zeros(num, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    t = 0.0
+    for k in range(args.warmup):
+        acc.integrate_device(num, t, t + dt, dt, y_dev.data_ptr(), pr_dev.data_ptr(), codes_dev.data_ptr(), sp)
+        t += dt
+    torch.cuda.synchronize()
+    acc.reset_counters()
+    torch.cuda.synchronize()
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        acc.integrate_device(num, t, t + dt, dt, y_dev.data_ptr(), pr_dev.data_ptr(), codes_dev.data_ptr(), sp)
+        ends[k].record(stream)
+        t += dt
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_ms = float(np.mean(launch_ms))
+
+    codes = codes_dev.cpu().numpy()
+    yf = y_dev.cpu().numpy()
+    cnt = acc.get_counters(num)
+    flops_total = algorithmic_flops(cnt, n, F, Jc)
+    flops_per_launch = flops_total / args.steps
+    achieved_tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    per_unit = cnt.sum(0) / (num * args.steps)
+    log("config %s: %s" % (args.config, desc))
+    log("per condition-step events: " + ", ".join("%s=%.2f" % (k, v) for k, v in zip(acc.CNT_NAMES, per_unit)))
+    log("launch ms: mean %.3f min %.3f max %.3f; wall %.3f s; nonzero codes %d; finite %s" % (
+        avg_ms, min(launch_ms), max(launch_ms), wall, int((codes != 0).sum()), bool(np.isfinite(yf).all())))
+    log("algorithmic FLOPs per condition-step %.3g" % (flops_total / (num * args.steps)))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, mech, y0, pr, dt)
+        log("cpu baseline: %s" % json.dumps(cpu))
+    value = num * args.steps / wall
+    out = {
+        "metric": "conditions/sec per 1e-6 s step" if n != 2 else "conditions/sec per 1e-3 s step",
+        "value": value,
+        "unit": "conditions*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "%s: %s" % (args.config, desc), "method": "plain FP64 Radau IIA-5",
+                   "rtol": 1e-6, "atol": 1e-10, "n_states": n, "conditions": num, "dt": dt},
+        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    acc.cleanup()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,135 @@
+/*
+ * accelerInt-compatible C ABI of the MI355X (gfx950) batched Radau IIA-5
+ * library, libaccelerint_hip.so.
+ *
+ * The first group of entry points has exactly the signatures of the
+ * reference's GPU library interface and solver hooks, so a caller written
+ * against SLACKHA/accelerInt's radau2a GPU build links against this library
+ * unchanged.  The second group is additive (no reference counterpart):
+ * runtime mechanism and tolerance selection, non-aborting status reporting,
+ * and a device-resident entry point whose buffers already live in HBM.
+ *
+ * Data layout at the boundary is the reference's: y_host is column-major,
+ * y_host[i * NUM + tid] holds state component i of condition tid
+ * (examples/main.dox:70-82, generic/solver_generic.c:41-44), and
+ * var_host[tid] is the per-condition parameter (mu for van der Pol, pressure
+ * in Pa for the kinetics networks).  y_host is updated in place.
+ */
+#ifndef ACCELERINT_H
+#define ACCELERINT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference interface ---------------------------------------------------- */
+
+/* Replaces generic/solver_interface.cuh:35 accelerInt_initialize(int NUM, int device).
+ * device < 0 selects device 0; an out-of-range device prints the reference's
+ * message and exits with status 1 (solver_interface.cu:97-107). */
+void accelerInt_initialize(int NUM, int device);
+
+/* Replaces generic/solver_interface.cuh:49-50.  Integrates NUM conditions from
+ * t_start to t_end in global steps of `stepsize` (stepsize < 0: one step of
+ * t_end - t_start).  Global step k ends at t_start + (k+1)*stepsize, clipped to
+ * t_end; the reference's own loop ignores t_start there
+ * (solver_interface.cu:209), which this library does not reproduce. */
+void accelerInt_integrate(const int NUM, const double t_start, const double t_end,
+                          const double stepsize, double *y_host, const double *var_host);
+
+/* Replaces generic/solver_interface.cuh:55 accelerInt_cleanup(). */
+void accelerInt_cleanup(void);
+
+/* Replaces the CPU driver generic/solver.h:32-33 intDriver(NUM, t, t_end, pr, y):
+ * one global step [t, t_end] of all NUM conditions, run on the GPU, same
+ * column-major y_global layout.  Initializes device 0 on first use. */
+void intDriver(const int NUM, const double t, const double t_end, const double *pr_global,
+               double *y_global);
+
+/* Solver hooks, generic/solver_init.cuh:21-26 and solver_props.cuh:25-26. */
+const char *solver_name(void);            /* "radau2a-int-gpu" (radau2a_init.cu:28-31) */
+size_t required_solver_size(void);        /* device bytes per condition, solver part */
+size_t required_mechanism_size(void);     /* device bytes per condition, mechanism part
+                                             (examples/van_der_pol/gpu_memory.cu:17-29) */
+void init_solver_log(void);
+void solver_log(void);
+/* radau2a_props.cu:15-40: prints the reference's message for the first
+ * condition with a non-zero code and exits with that code. */
+void check_error(int num_conditions, int *code_arr);
+
+/* ---- additive interface (no reference counterpart) ------------------------ */
+
+/* Mechanism selection; must precede accelerInt_initialize.  The reference
+ * compiles exactly one mechanism into each build (examples/van_der_pol/header.h:26,
+ * mechanism/ for pyJac output); this library carries the van der Pol example
+ * and the synthetic constant-pressure kinetics family, chosen at run time. */
+#define ACCELERINT_MECH_VAN_DER_POL 0
+#define ACCELERINT_MECH_KINETICS 2
+int accelerInt_select_mechanism(int kind);
+
+/* Synthetic kinetics network (state (T, Y_1..Y_ns), N = ns + 1).  Arrays are
+ * copied.  slots[6*j + 0..2] = reactant species (-1 = empty), slots[6*j + 3..5]
+ * = product species; thermo[5*s + 0..4] = a0, a1, a2, bh, bs with
+ * cp/R = a0 + a1 T + a2 T^2, h/R = a0 T + a1 T^2/2 + a2 T^3/3 + bh,
+ * s/R = a0 ln T + a1 T + a2 T^2/2 + bs; rate constants
+ * kf = exp(lnA + beta ln(T/1000) - EaR/T) in mol-cm-s units.
+ * Returns 0, or -1 if ns + 1 is not a compiled system size. */
+int accelerInt_set_kinetics(int ns, int nr, const double *W, const double *thermo,
+                            const int *slots, const double *lnA, const double *beta,
+                            const double *EaR);
+
+/* Runtime tolerances (the reference's ATOL/RTOL macros; defaults 1e-10, 1e-6)
+ * and the CONST_TIME_STEP switch. */
+int accelerInt_set_tolerances(double atol, double rtol);
+int accelerInt_set_const_time_step(int on);
+
+/* 1 (default): any non-zero condition code aborts the process through
+ * check_error, as the reference does.  0: codes are kept and returned by
+ * accelerInt_get_status; integration continues for the other conditions. */
+int accelerInt_set_abort_on_error(int on);
+
+/* Codes of the last integrate call (num entries); returns the count of
+ * non-zero codes, or -1 if no call has been made. */
+int accelerInt_get_status(int *codes, int num);
+
+/* Per-condition event counters accumulated since initialize:
+ * out[c*10 + k], k = step, accept, reject, newton_fail, lu, lu_fail, newton,
+ * f, jac, es.  Returns 0 on success. */
+int accelerInt_get_counters(uint32_t *out, int num);
+int accelerInt_reset_counters(void);
+
+/* System size of the selected mechanism. */
+int accelerInt_num_states(void);
+
+/* Device-resident integration: y_dev is condition-major (y_dev[c*N + i]),
+ * var_dev[c], codes_dev[c] (may be NULL) are device pointers in HBM; `stream`
+ * is a hipStream_t (NULL = the library's stream).  Runs the same global-step
+ * loop as accelerInt_integrate without host copies and returns without
+ * waiting for completion.  Returns 0 or a negative error. */
+int accelerInt_integrate_device(int NUM, double t_start, double t_end, double stepsize,
+                                double *y_dev, const double *var_dev, int *codes_dev,
+                                void *stream);
+
+/* Evaluates the mechanism callbacks on the device at each condition's state
+ * without integrating (the reference's rate/Jacobian dump,
+ * tests/rateOutputTest.cu:18): y_host/var_host as in accelerInt_integrate,
+ * dy_host column-major like y_host, jac_host[c*N*N + i + j*N] the column-major
+ * Jacobian of condition c.  NUM must not exceed the initialized capacity.
+ * Returns 0 or a negative error. */
+int accelerInt_eval_callbacks(int NUM, double t, const double *y_host, const double *var_host,
+                              double *dy_host, double *jac_host);
+
+/* Blocks until the library stream is idle; returns 0 or a negative error. */
+int accelerInt_synchronize(void);
+
+/* Last error message recorded by the library ("" if none). */
+const char *accelerInt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,219 @@
+"""CPU oracle for the batched Radau IIA-5 path -- TEST INFRASTRUCTURE ONLY.
+
+This package is the checker: only ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` may import it.  The product library
+(``accelerint_amd``) never imports, links or calls anything here.
+
+``liboracle.so`` is a C restatement of the reference's CPU integrator
+(radau2a/radau2a.c:37-736) and OpenMP driver (generic/solver_generic.c:28-58),
+with LAPACK-semantics LU/solves (generic/lapack_dfns.h:12-15) and the
+callbacks of examples/van_der_pol (dydt.c:28-35, jacob.c:29-41), a linear test
+system, and the synthetic kinetics network.
+
+Pinning status of each function (what checks it independently of itself) is
+listed in ``tests/test_oracle_*.py``; see the module docstrings there.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+NCNT = 10
+CNT_NAMES = ("step", "accept", "reject", "newton_fail", "lu", "lu_fail", "newton", "f", "jac", "es")
+MECH_VDP, MECH_LINEAR, MECH_KINETICS = 0, 1, 2
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile (gcc, OpenMP)."""
+    args = ["make", "-s", "-C", _HERE]
+    if force:
+        args.insert(1, "-B")
+    subprocess.run(args, check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        lp = ctypes.POINTER(ctypes.c_int64)
+        vp = ctypes.c_void_p
+        L.ora_batch.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                dp, dp, ip, lp, ctypes.c_int]
+        L.ora_batch.restype = None
+        L.ora_eval.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                               ctypes.c_double, dp, dp]
+        L.ora_fd_jacob.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.ora_dgetrf.argtypes = [ctypes.c_int, dp, ip, ip]
+        L.ora_zgetrf.argtypes = [ctypes.c_int, dp, ip, ip]
+        L.ora_dgetrs.argtypes = [ctypes.c_int, dp, ip, dp]
+        L.ora_zgetrs.argtypes = [ctypes.c_int, dp, ip, dp]
+        L.ora_kin_rates.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class _OraLinear(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("m", ctypes.POINTER(ctypes.c_double))]
+
+
+class _OraKinetics(ctypes.Structure):
+    _fields_ = [("ns", ctypes.c_int), ("nr", ctypes.c_int),
+                ("W", ctypes.POINTER(ctypes.c_double)),
+                ("thermo", ctypes.POINTER(ctypes.c_double)),
+                ("slots", ctypes.POINTER(ctypes.c_int)),
+                ("lnA", ctypes.POINTER(ctypes.c_double)),
+                ("beta", ctypes.POINTER(ctypes.c_double)),
+                ("EaR", ctypes.POINTER(ctypes.c_double))]
+
+
+class Mechanism:
+    """A callback set the oracle can integrate: kind + context + system size."""
+
+    def __init__(self, kind: int, n: int, ctx=None, keep=()):
+        self.kind, self.n, self._ctx, self._keep = kind, n, ctx, keep
+
+    @property
+    def ctx_ptr(self):
+        return None if self._ctx is None else ctypes.cast(ctypes.pointer(self._ctx), ctypes.c_void_p)
+
+    @classmethod
+    def van_der_pol(cls):
+        return cls(MECH_VDP, 2)
+
+    @classmethod
+    def linear(cls, m):
+        m = np.asfortranarray(np.asarray(m, dtype=np.float64))
+        flat = np.ascontiguousarray(m.reshape(-1, order="F"))
+        ctx = _OraLinear(m.shape[0], _dp(flat))
+        return cls(MECH_LINEAR, m.shape[0], ctx, keep=(flat,))
+
+    @classmethod
+    def kinetics(cls, mech):
+        """``mech`` is an ``accelerint_amd.mechanism.KineticsMechanism``-like object."""
+        W = np.ascontiguousarray(mech.W, dtype=np.float64)
+        th = np.ascontiguousarray(mech.thermo, dtype=np.float64).reshape(-1)
+        sl = np.ascontiguousarray(mech.slots, dtype=np.int32).reshape(-1)
+        lnA = np.ascontiguousarray(mech.lnA, dtype=np.float64)
+        beta = np.ascontiguousarray(mech.beta, dtype=np.float64)
+        EaR = np.ascontiguousarray(mech.EaR, dtype=np.float64)
+        ctx = _OraKinetics(int(mech.ns), int(mech.nr), _dp(W), _dp(th), _ip(sl), _dp(lnA),
+                           _dp(beta), _dp(EaR))
+        return cls(MECH_KINETICS, int(mech.ns) + 1, ctx, keep=(W, th, sl, lnA, beta, EaR))
+
+
+def integrate(mech: Mechanism, t, t_end, pr, y_colmajor, atol=1e-10, rtol=1e-6,
+              const_time_step=False, nthreads=0):
+    """One intDriver call (solver_generic.c:28): y is column-major (n, num),
+    updated copy returned with codes and counters."""
+    y = np.array(y_colmajor, dtype=np.float64, order="C", copy=True)
+    n, num = y.shape
+    assert n == mech.n
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    codes = np.zeros(num, dtype=np.int32)
+    cnt = np.zeros((num, NCNT), dtype=np.int64)
+    lib().ora_batch(mech.kind, mech.ctx_ptr, n, atol, rtol, int(const_time_step), num, t, t_end,
+                    _dp(pr), _dp(y), _ip(codes),
+                    cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(nthreads))
+    return y, codes, cnt
+
+
+def integrate_steps(mech, y0, pr, dt, nsteps, t0=0.0, **kw):
+    """Global-step loop of solver_main.c:202-209 (t_next = (k+1)*dt)."""
+    y = np.array(y0, dtype=np.float64, copy=True)
+    codes_all = np.zeros(y.shape[1], dtype=np.int32)
+    cnt_all = np.zeros((y.shape[1], NCNT), dtype=np.int64)
+    t = t0
+    for k in range(nsteps):
+        t_next = t0 + (k + 1) * dt
+        y, codes, cnt = integrate(mech, t, t_next, pr, y, **kw)
+        codes_all = np.where(codes_all == 0, codes, codes_all)
+        cnt_all += cnt
+        t = t_next
+    return y, codes_all, cnt_all
+
+
+def dydt(mech, t, pr, y):
+    out = np.zeros(mech.n)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    lib().ora_eval(mech.kind, mech.ctx_ptr, mech.n, 0, t, pr, _dp(y), _dp(out))
+    return out
+
+
+def jacobian(mech, t, pr, y):
+    out = np.zeros(mech.n * mech.n)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    lib().ora_eval(mech.kind, mech.ctx_ptr, mech.n, 1, t, pr, _dp(y), _dp(out))
+    return out.reshape(mech.n, mech.n, order="F")
+
+
+def fd_jacobian(mech, t, pr, y, atol=1e-10, rtol=1e-6):
+    out = np.zeros(mech.n * mech.n)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    lib().ora_fd_jacob(mech.kind, mech.ctx_ptr, mech.n, atol, rtol, t, pr, _dp(y), _dp(out))
+    return out.reshape(mech.n, mech.n, order="F")
+
+
+def kinetics_rates(mech, T, P, Y):
+    q = np.zeros(mech._ctx.nr)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    lib().ora_kin_rates(mech.ctx_ptr, T, P, _dp(Y), _dp(q))
+    return q
+
+
+def dgetrf(a):
+    a = np.asfortranarray(np.array(a, dtype=np.float64))
+    n = a.shape[0]
+    flat = np.ascontiguousarray(a.reshape(-1, order="F"))
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = np.zeros(1, dtype=np.int32)
+    lib().ora_dgetrf(n, _dp(flat), _ip(ipiv), _ip(info))
+    return flat.reshape(n, n, order="F"), ipiv, int(info[0])
+
+
+def zgetrf(a):
+    a = np.asarray(a, dtype=np.complex128)
+    n = a.shape[0]
+    flat = np.ascontiguousarray(a.reshape(-1, order="F")).view(np.float64).copy()
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = np.zeros(1, dtype=np.int32)
+    lib().ora_zgetrf(n, _dp(flat), _ip(ipiv), _ip(info))
+    return flat.view(np.complex128).reshape(n, n, order="F"), ipiv, int(info[0])
+
+
+def dgetrs(lu, ipiv, b):
+    n = lu.shape[0]
+    flat = np.ascontiguousarray(np.asarray(lu, dtype=np.float64).reshape(-1, order="F"))
+    x = np.array(b, dtype=np.float64, copy=True)
+    lib().ora_dgetrs(n, _dp(flat), _ip(np.ascontiguousarray(ipiv, dtype=np.int32)), _dp(x))
+    return x
+
+
+def zgetrs(lu, ipiv, b):
+    n = lu.shape[0]
+    flat = np.ascontiguousarray(np.asarray(lu, dtype=np.complex128).reshape(-1, order="F"))
+    flat = flat.view(np.float64).copy()
+    x = np.array(b, dtype=np.complex128, copy=True).view(np.float64).copy()
+    lib().ora_zgetrs(n, _dp(flat), _ip(np.ascontiguousarray(ipiv, dtype=np.int32)), _dp(x))
+    return x.view(np.complex128)

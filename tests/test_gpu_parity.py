@@ -1,0 +1,140 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: identical return codes and, per state component,
+|y_gpu - y_oracle| <= 10 * (atol + rtol * |y_oracle|)  ("within 10x rtol",
+the weighted metric of the reference's logger.py:53-55); callbacks to 1e-12
+relative; closed forms to 1e-12.
+"""
+import numpy as np
+import pytest
+
+import accelerint_amd as acc
+from accelerint_amd import mechanism as M
+
+pytestmark = pytest.mark.gpu
+
+ATOL, RTOL = 1e-10, 1e-6
+
+
+def weighted_err(y, yref, atol=ATOL, rtol=RTOL):
+    return np.abs(y - yref) / (atol + rtol * np.abs(yref))
+
+
+@pytest.fixture
+def session():
+    acc.lib()
+    yield acc
+    acc.cleanup()
+
+
+def _vdp_setup(num):
+    acc.select_mechanism(acc.MECH_VAN_DER_POL)
+    acc.set_tolerances(ATOL, RTOL)
+    acc.set_const_time_step(False)
+    acc.set_abort_on_error(False)
+    acc.initialize(num, 0)
+
+
+def test_vdp_config0_parity(session, oracle_lib):
+    """van der Pol, 4096 conditions, 100 global steps of 1e-3 (mu = 10^U(1,3))."""
+    num, steps, dt = 4096, 100, 1e-3
+    y0, mu = M.van_der_pol_conditions(num, seed=1234)
+    _vdp_setup(num)
+    y = y0.copy()
+    acc.integrate(num, 0.0, steps * dt, dt, y, mu)
+    codes = acc.get_status(num)
+    om = oracle_lib.Mechanism.van_der_pol()
+    yo, co, cnt_o = oracle_lib.integrate_steps(om, y0, mu, dt, steps, nthreads=8)
+    assert (codes == co).all()
+    assert (codes == 0).all()
+    we = weighted_err(y, yo)
+    print("vdp weighted err median %.3g max %.3g" % (np.median(we), we.max()))
+    assert we.max() <= 10.0
+    cnt_g = acc.get_counters(num).astype(np.int64)
+    same = (cnt_g == cnt_o).all(1).mean()
+    print("conditions with identical event counters: %.3f" % same)
+    assert same > 0.5
+
+
+def test_vdp_int_driver_matches_integrate(session):
+    num = 300
+    y0, mu = M.van_der_pol_conditions(num, seed=7)
+    _vdp_setup(num)
+    ya = y0.copy()
+    acc.integrate(num, 0.0, 2e-3, 1e-3, ya, mu)
+    yb = y0.copy()
+    acc.int_driver(num, 0.0, 1e-3, mu, yb)
+    acc.int_driver(num, 1e-3, 2e-3, mu, yb)
+    assert np.array_equal(ya, yb)
+
+
+def test_harmonic_const_step_closed_form(session):
+    """mu = 0 turns van der Pol into y'' = -y; with CONST_TIME_STEP every
+    global step is one Radau step, so y_n = R(hA)^n y_0 with the Radau IIA
+    stability function R(z) = (1 + 2z/5 + z^2/20) / (1 - 3z/5 + 3z^2/20 - z^3/60)."""
+    num, n, h = 64, 25, 0.3
+    rng = np.random.default_rng(3)
+    y0 = np.ascontiguousarray(rng.uniform(-1, 1, (2, num)))
+    mu = np.zeros(num)
+    acc.select_mechanism(acc.MECH_VAN_DER_POL)
+    acc.set_tolerances(ATOL, RTOL)
+    acc.set_const_time_step(True)
+    acc.set_abort_on_error(False)
+    acc.initialize(num, 0)
+    y = y0.copy()
+    acc.integrate(num, 0.0, n * h, h, y, mu)
+    acc.set_const_time_step(False)
+    A = np.array([[0.0, 1.0], [-1.0, 0.0]])
+    Z = h * A
+    I = np.eye(2)
+    num_p = I + 2 * Z / 5 + Z @ Z / 20
+    den_p = I - 3 * Z / 5 + 3 * Z @ Z / 20 - Z @ Z @ Z / 60
+    Rm = np.linalg.solve(den_p, num_p)
+    yref = np.linalg.matrix_power(Rm, n) @ y0
+    assert np.abs(y - yref).max() < 1e-12
+
+
+@pytest.fixture(scope="module")
+def net14():
+    return M.network(14)
+
+
+def _kin_setup(mech, num):
+    acc.set_kinetics(mech)
+    acc.set_tolerances(ATOL, RTOL)
+    acc.set_const_time_step(False)
+    acc.set_abort_on_error(False)
+    acc.initialize(num, 0)
+
+
+def test_kinetics14_callbacks(session, oracle_lib, net14):
+    num = 512
+    y0, P = M.initial_conditions(net14, num, seed=11)
+    _kin_setup(net14, num)
+    dy, jac = acc.eval_callbacks(num, 0.0, y0, P)
+    om = oracle_lib.Mechanism.kinetics(net14)
+    for c in range(0, num, 37):
+        fo = oracle_lib.dydt(om, 0.0, P[c], y0[:, c])
+        Jo = oracle_lib.jacobian(om, 0.0, P[c], y0[:, c])
+        assert np.allclose(dy[:, c], fo, rtol=1e-11, atol=1e-12 * np.abs(fo).max())
+        scale = np.abs(Jo).max(1, keepdims=True)
+        assert (np.abs(jac[c] - Jo) <= 1e-11 * scale).all()
+
+
+def test_kinetics14_parity(session, oracle_lib, net14):
+    """14-state network, 1024 conditions, 20 global steps of 1e-6 s."""
+    num, steps, dt = 1024, 20, 1e-6
+    y0, P = M.initial_conditions(net14, num, seed=20261202)
+    _kin_setup(net14, num)
+    y = y0.copy()
+    acc.integrate(num, 0.0, steps * dt, dt, y, P)
+    codes = acc.get_status(num)
+    om = oracle_lib.Mechanism.kinetics(net14)
+    yo, co, cnt_o = oracle_lib.integrate_steps(om, y0, P, dt, steps, nthreads=8)
+    assert (codes == co).all()
+    assert np.isfinite(y).all()
+    we = weighted_err(y, yo)
+    print("kinetics14 weighted err median %.3g p99 %.3g max %.3g" % (np.median(we), np.percentile(we, 99), we.max()))
+    assert we.max() <= 10.0
+    cnt_g = acc.get_counters(num).astype(np.int64)
+    print("conditions with identical event counters: %.3f" % (cnt_g == cnt_o).all(1).mean())
