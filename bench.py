@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--num", type=int, default=0, help="override condition count")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (rocprof traffic calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the integrator shards by condition; "
@@ -139,8 +141,12 @@ def main():
     y_dev = torch.from_numpy(np.ascontiguousarray(y0.T)).to(dev)  # condition-major
     pr_dev = torch.from_numpy(pr).to(dev)
     codes_dev = torch.zeros(num, dtype=torch.int32, device=dev)
-    stream = torch.cuda.current_stream(dev)
+    torch.cuda.synchronize()
+    # one explicit (default-priority) stream for the library's launches and the
+    # timing events, so the events bracket exactly the integration kernels
+    stream = torch.cuda.Stream(device=dev)
     sp = stream.cuda_stream
+    assert sp != 0
 
     t = 0.0
     for k in range(args.warmup):
@@ -164,6 +170,14 @@ def main():
     launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     avg_ms = float(np.mean(launch_ms))
 
+    if args.calib_copy_mb:
+        # a plain copy of known size, for scaling rocprof FETCH_SIZE/WRITE_SIZE
+        src = torch.ones(args.calib_copy_mb * (1 << 20) // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mb, args.calib_copy_mb))
     codes = codes_dev.cpu().numpy()
     yf = y_dev.cpu().numpy()
     cnt = acc.get_counters(num)

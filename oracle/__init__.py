@@ -62,6 +62,7 @@ def lib():
         L.ora_dgetrs.argtypes = [ctypes.c_int, dp, ip, dp]
         L.ora_zgetrs.argtypes = [ctypes.c_int, dp, ip, dp]
         L.ora_kin_rates.argtypes = [vp, ctypes.c_double, ctypes.c_double, dp, dp]
+        L.ora_tableau.argtypes = [dp]
         _lib = L
     return _lib
 
@@ -180,6 +181,15 @@ def kinetics_rates(mech, T, P, Y):
     Y = np.ascontiguousarray(Y, dtype=np.float64)
     lib().ora_kin_rates(mech.ctx_ptr, T, P, _dp(Y), _dp(q))
     return q
+
+
+def tableau():
+    """The oracle's Radau IIA-5 constants as a dict of numpy arrays."""
+    out = np.zeros(45)
+    lib().ora_tableau(_dp(out))
+    return {"A": out[0:9].reshape(3, 3), "c": out[9:12], "e": out[12:16], "gamma": out[16],
+            "alpha": out[17], "beta": out[18], "T": out[19:28].reshape(3, 3),
+            "TinvAinv": out[28:37].reshape(3, 3), "ELO": out[37]}
 
 
 def dgetrf(a):

@@ -70,6 +70,10 @@ void ora_zgetrf(int n, double *a /* interleaved re,im */, int *ipiv, int *info);
 void ora_dgetrs(int n, const double *a, const int *ipiv, double *b);
 void ora_zgetrs(int n, const double *a, const int *ipiv, double *b /* interleaved */);
 
+/* Tableau constants (radau2a.c:101-231) in the order A, c, e, gamma, alpha,
+ * beta, T, TinvAinv, ELO (45 doubles). */
+void ora_tableau(double *out);
+
 /* radau2a.c:520 integrate(); counters may be NULL. */
 int ora_integrate(const ora_problem *p, double t_start, double t_end, double pr, double *y,
                   int64_t *counters);

@@ -65,6 +65,20 @@ static const double rkTinvAinv[3][3] = {
      -1.441236197545344702389881889085515e-1}};
 static const double rkELO = 4;
 
+/* Copies the tableau into out[0..44]: A (9, row-major), c (3), e (4),
+ * gamma, alpha, beta, T (9), TinvAinv (9), ELO, so tests can check every
+ * constant against its closed form. */
+void ora_tableau(double *out) {
+    int k = 0;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[k++] = rkA[i][j];
+    for (int i = 0; i < 3; ++i) out[k++] = rkC[i];
+    for (int i = 0; i < 4; ++i) out[k++] = rkE[i];
+    out[k++] = rkGamma; out[k++] = rkAlpha; out[k++] = rkBeta;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[k++] = rkT[i][j];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[k++] = rkTinvAinv[i][j];
+    out[k++] = rkELO;
+}
+
 /* ---- LAPACK semantics ---------------------------------------------------- */
 
 /* Smith's complex division (a+ib)/(c+id). */

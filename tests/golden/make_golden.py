@@ -1,0 +1,57 @@
+"""Writes the tight-tolerance reference solutions used by the tests.
+
+The reference's own error oracle is a tight CVODES run (logger.py:164-178,
+cvodes/); SUNDIALS is not available here, so the substitute is scipy's
+solve_ivp (Radau for van der Pol, BDF for the kinetics network) at
+rtol 1e-12 / 1e-11 with the same right-hand sides, run in this container.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+from scipy.integrate import solve_ivp
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from accelerint_amd import mechanism as M  # noqa: E402
+import oracle  # noqa: E402  (callback evaluation only)
+
+
+def vdp():
+    num, steps, dt = 32, 100, 1e-3
+    y0, mu = M.van_der_pol_conditions(4096, seed=1234)
+    y0, mu = y0[:, :num].copy(), mu[:num].copy()
+    out = np.zeros_like(y0)
+    for c in range(num):
+        f = lambda t, y, m=mu[c]: [y[1], m * (1 - y[0] ** 2) * y[1] - y[0]]
+        J = lambda t, y, m=mu[c]: [[0.0, 1.0], [-2 * m * y[0] * y[1] - 1, m * (1 - y[0] ** 2)]]
+        s = solve_ivp(f, (0.0, steps * dt), y0[:, c], method="Radau", rtol=1e-12, atol=1e-14, jac=J)
+        assert s.success
+        out[:, c] = s.y[:, -1]
+    np.savez(os.path.join(HERE, "vdp_tight.npz"), y0=y0, mu=mu, y_tight=out, dt=dt, steps=steps,
+             solver="scipy Radau rtol 1e-12 atol 1e-14")
+
+
+def kinetics(n, num, steps, dt, seed, T_range):
+    mech = M.network(n)
+    om = oracle.Mechanism.kinetics(mech)
+    y0, P = M.initial_conditions(mech, num, seed=seed, T_range=T_range)
+    out = np.zeros_like(y0)
+    for c in range(num):
+        f = lambda t, y, p=P[c]: oracle.dydt(om, t, p, y)
+        J = lambda t, y, p=P[c]: oracle.jacobian(om, t, p, y)
+        s = solve_ivp(f, (0.0, steps * dt), y0[:, c], method="BDF", rtol=1e-11, atol=1e-16, jac=J)
+        assert s.success, s.message
+        out[:, c] = s.y[:, -1]
+    np.savez(os.path.join(HERE, "kinetics%d_tight.npz" % n), y0=y0, P=P, y_tight=out, dt=dt, steps=steps,
+             seed=seed, solver="scipy BDF rtol 1e-11 atol 1e-16")
+
+
+if __name__ == "__main__":
+    vdp()
+    kinetics(14, 16, 20, 1e-6, 99, (1000.0, 1600.0))
+    kinetics(54, 4, 5, 1e-6, 98, (800.0, 2200.0))
+    print("golden fixtures written")

@@ -1,0 +1,55 @@
+"""The C ABI library loads and exports every symbol include/accelerint.h
+declares (no GPU needed: nothing is called that touches a device)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import accelerint_amd as acc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "accelerint.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", src, flags=re.M)
+    return sorted(set(n for n in names if n not in ("defined",)))
+
+
+def test_header_parsed():
+    syms = declared_symbols()
+    assert "accelerInt_initialize" in syms and "intDriver" in syms and len(syms) >= 20
+
+
+def test_library_exports_declared_symbols():
+    if not os.path.exists(acc.LIB_PATH):
+        pytest.skip("libaccelerint_hip.so not built")
+    L = ctypes.CDLL(acc.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(acc.EXPORTS) == declared_symbols()
+
+
+def test_solver_name_and_sizes_without_gpu():
+    if not os.path.exists(acc.LIB_PATH):
+        pytest.skip("libaccelerint_hip.so not built")
+    assert acc.solver_name() == "radau2a-int-gpu"
+    assert acc.required_solver_size() > 0
+
+
+def test_bad_configuration_is_rejected():
+    if not os.path.exists(acc.LIB_PATH):
+        pytest.skip("libaccelerint_hip.so not built")
+    with pytest.raises(acc.AccelerIntError):
+        acc.set_tolerances(-1.0, 1e-6)
+    with pytest.raises(acc.AccelerIntError):
+        acc.select_mechanism(7)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(acc, "_lib", None)
+    monkeypatch.setattr(acc, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(acc.AccelerIntError):
+        acc.lib()
