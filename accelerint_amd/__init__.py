@@ -20,7 +20,7 @@ import numpy as np
 from . import mechanism  # noqa: F401  (re-export)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libaccelerint_hip.so")
+LIB_PATH = os.environ.get("ACCELERINT_LIB") or os.path.join(_HERE, "libaccelerint_hip.so")
 _lib = None
 
 MECH_VAN_DER_POL = 0

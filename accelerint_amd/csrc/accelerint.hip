@@ -25,51 +25,96 @@
 #include "mech_vdp.hpp"
 #include "radau_core.hpp"
 
+// Waves per SIMD the register allocator targets for each instantiation.
+#ifndef ACC_VDP_WPE
+#define ACC_VDP_WPE 2
+#endif
+#ifndef ACC_K14_WPE
+#define ACC_K14_WPE 2
+#endif
+
 namespace acc {
 
 constexpr int kBlock = 256;
 
-// One global step [t, t_end] for every condition: the analogue of the
-// reference's intDriver kernel (solver_generic.cu:29-42), one lane group per
-// condition instead of one thread.
+// The integration kernel: the analogue of the reference's per-step intDriver
+// launch (solver_generic.cu:29-42) wrapped in its host global-step loop
+// (solver_interface.cu:234-274), turned inside out.  Conditions are
+// independent, so one launch runs every global step of a condition back to
+// back; lane groups are persistent and pull condition indices from an atomic
+// work counter.  A group advances one step attempt per loop iteration, so the
+// moment its condition (or global step) ends it starts the next one -- no
+// group idles waiting for a slow neighbour and no grid-wide barrier separates
+// global steps.
+template <int G>
+__device__ __forceinline__ int grab_condition(int* work) {
+  int c = 0;
+  if (Grp<G>::lane() == 0) c = atomicAdd(work, 1);
+  return Grp<G>::bcast_i(c, 0);
+}
+
 template <class Mech, int G, int R, int BLOCK, int WPE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau_step(int num, double t, double t_end,
-                                                       const double* __restrict__ pr,
-                                                       double* __restrict__ y,
-                                                       int* __restrict__ codes,
-                                                       uint32_t* __restrict__ counters,
-                                                       SolverParams P, typename Mech::Params mp) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
+    int num, double t_start, double t_end, double step, const double* __restrict__ pr,
+    double* __restrict__ y, int* __restrict__ codes, uint32_t* __restrict__ counters,
+    int* __restrict__ work, SolverParams P, typename Mech::Params mp) {
   extern __shared__ double lds_dyn[];
   constexpr int N = Mech::N;
-  constexpr int GPB = BLOCK / G;
   const int g = threadIdx.x / G;
-  const int c = blockIdx.x * GPB + g;
-  if (c >= num) return;
   const int ln = threadIdx.x % G;
-  double yl[R];
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int r = ln + G * q;
-    yl[q] = (r < N) ? y[(size_t)c * N + r] : 0.0;
-  }
+  Mech::block_init(mp, lds_dyn);
   typename Mech::Ctx ctx(mp, g, lds_dyn);
+  Radau<Mech, G, R> st;
   uint32_t cnt[NCNT];
+  double prc = 0.0, t_next = t_end;
+  int k = 0;
+  int c = grab_condition<G>(work);
+  bool fresh = true;
+#pragma unroll 1
+  while (c < num) {
+    if (fresh) {  // load condition c and open its first global step
 #pragma unroll
-  for (int k = 0; k < NCNT; ++k) cnt[k] = 0;
-  const int code = Radau<Mech, G, R>::integrate(ctx, P, t, t_end, pr[c], yl, cnt);
+      for (int q = 0; q < R; ++q) {
+        const int r = ln + G * q;
+        st.y[q] = (r < N) ? y[(size_t)c * N + r] : 0.0;
+      }
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int r = ln + G * q;
-    if (r < N) y[(size_t)c * N + r] = yl[q];
-  }
-  if (ln == 0) {
-    if (codes) codes[c] = code;
-    if (counters) {
-#pragma unroll
-      for (int k = 0; k < NCNT; ++k) counters[(size_t)c * NCNT + k] += cnt[k];
+      for (int i = 0; i < NCNT; ++i) cnt[i] = 0;
+      prc = pr[c];
+      k = 0;
+      t_next = fmin(t_end, t_start + step);
+      st.begin(P, t_start, t_next);
+      fresh = false;
     }
+    int code = EC_success;
+    if (!st.finished()) code = st.attempt(ctx, P, prc, cnt);
+    if (code == RADAU_CONTINUE) continue;
+    if (code == EC_success && t_next + kRoundoff < t_end) {  // next global step
+      const double t0 = t_next;
+      ++k;
+      t_next = fmin(t_end, t_start + (k + 1) * step);
+      st.begin(P, t0, t_next);
+      continue;
+    }
+    // condition finished (all steps done, or an error code): write back
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = ln + G * q;
+      if (r < N) y[(size_t)c * N + r] = st.y[q];
+    }
+    if (ln == 0) {
+      if (codes) codes[c] = code;
+      if (counters) {
+#pragma unroll
+        for (int i = 0; i < NCNT; ++i) counters[(size_t)c * NCNT + i] += cnt[i];
+      }
+    }
+    c = grab_condition<G>(work);
+    fresh = true;
   }
 }
+
+__global__ void k_reset_work(int* work) { *work = 0; }
 
 // Evaluates the mechanism callbacks at every condition's state (no
 // integration): dy[c*N + i] and jac[c*N*N + i + j*N] (column-major per
@@ -87,6 +132,7 @@ __global__ __launch_bounds__(BLOCK) void k_eval_callbacks(int num, double t,
   constexpr int GPB = BLOCK / G;
   const int g = threadIdx.x / G;
   const int c = blockIdx.x * GPB + g;
+  Mech::block_init(mp, lds_dyn);
   if (c >= num) return;
   const int ln = threadIdx.x % G;
   double yl[R], f[R], J[R][N];
@@ -136,15 +182,31 @@ __global__ void k_cond_to_cm(const double* __restrict__ src, double* __restrict_
   dst[(size_t)i * ld + c] = src[(size_t)c * n + i];
 }
 
+// Persistent grid: as many blocks as fit on the device at once (capped by the
+// number of conditions).
 template <class Mech, int G, int R, int BLOCK, int WPE>
-hipError_t launch_step(int num, double t, double t_end, const double* pr, double* y, int* codes,
-                       uint32_t* cnt, const SolverParams& P, const typename Mech::Params& mp,
-                       hipStream_t s) {
+hipError_t launch_radau(int num, double t_start, double t_end, double step, const double* pr,
+                        double* y, int* codes, uint32_t* cnt, int* work, const SolverParams& P,
+                        const typename Mech::Params& mp, hipStream_t s) {
   constexpr int GPB = BLOCK / G;
-  const int blocks = (num + GPB - 1) / GPB;
-  if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_radau_step<Mech, G, R, BLOCK, WPE>), dim3(blocks), dim3(BLOCK),
-                     Mech::lds_bytes(mp, GPB), s, num, t, t_end, pr, y, codes, cnt, P, mp);
+  if (num <= 0) return hipSuccess;
+  const size_t lds = Mech::lds_bytes(mp, GPB);
+  static int resident = 0;  // blocks per device that fit at once, per instantiation
+  if (resident == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_radau<Mech, G, R, BLOCK, WPE>,
+                                                       BLOCK, lds);
+    if (e != hipSuccess) return e;
+    resident = (cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+  }
+  int blocks = (num + GPB - 1) / GPB;
+  if (blocks > resident) blocks = resident;
+  hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work);
+  hipLaunchKernelGGL((k_radau<Mech, G, R, BLOCK, WPE>), dim3(blocks), dim3(BLOCK), lds, s, num,
+                     t_start, t_end, step, pr, y, codes, cnt, work, P, mp);
   return hipGetLastError();
 }
 
@@ -168,6 +230,7 @@ struct State {
   double* d_pr = nullptr;
   int* d_codes = nullptr;
   uint32_t* d_cnt = nullptr;
+  int* d_work = nullptr;    // atomic work counter of the persistent kernel
   std::vector<int> h_codes;
   int have_status = 0;
   // kinetics tables (device copies)
@@ -210,6 +273,8 @@ void free_buffers() {
   if (S.d_pr) hipFree(S.d_pr);
   if (S.d_codes) hipFree(S.d_codes);
   if (S.d_cnt) hipFree(S.d_cnt);
+  if (S.d_work) hipFree(S.d_work);
+  S.d_work = nullptr;
   S.d_y = S.d_ycm = S.d_pr = nullptr;
   S.d_codes = nullptr;
   S.d_cnt = nullptr;
@@ -224,19 +289,22 @@ bool alloc_buffers(int cap) {
   if (!HIPOK(hipMalloc(&S.d_codes, (size_t)cap * sizeof(int)))) return false;
   if (!HIPOK(hipMalloc(&S.d_cnt, (size_t)cap * NCNT * sizeof(uint32_t)))) return false;
   if (!HIPOK(hipMemset(S.d_cnt, 0, (size_t)cap * NCNT * sizeof(uint32_t)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_work, sizeof(int)))) return false;
   S.padded = cap;
   S.h_codes.assign(cap, 0);
   return true;
 }
 
-hipError_t launch(int num, double t, double t_end, const double* pr, double* y, int* codes,
-                  uint32_t* cnt, hipStream_t s) {
+hipError_t launch(int num, double t_start, double t_end, double step, const double* pr, double* y,
+                  int* codes, uint32_t* cnt, hipStream_t s) {
   switch (S.mech) {
     case ACCELERINT_MECH_VAN_DER_POL:
-      return launch_step<MechVdP, 1, 2, 256, 2>(num, t, t_end, pr, y, codes, cnt, S.P, MechVdP::Params{}, s);
+      return launch_radau<MechVdP, 1, 2, 256, ACC_VDP_WPE>(num, t_start, t_end, step, pr, y, codes, cnt, S.d_work,
+                                                 S.P, MechVdP::Params{}, s);
     case ACCELERINT_MECH_KINETICS:
       if (S.n == 14)
-        return launch_step<MechKinetics14, 16, 1, 128, 2>(num, t, t_end, pr, y, codes, cnt, S.P, S.kp14, s);
+        return launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE>(num, t_start, t_end, step, pr, y, codes,
+                                                           cnt, S.d_work, S.P, S.kp14, s);
       return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
@@ -257,21 +325,13 @@ hipError_t launch_eval_any(int num, double t, const double* pr, const double* y,
 }
 
 // Global-step loop of solver_interface.cu:225-275 on device-resident,
-// condition-major buffers.
+// condition-major buffers: one launch covers every global step.
 int run_steps(int num, double t_start, double t_end, double stepsize, double* y, const double* pr,
               int* codes, uint32_t* cnt, hipStream_t s) {
   const double step = stepsize < 0 ? t_end - t_start : stepsize;
-  double t = t_start;
-  double t_next = fmin(t_end, t + step);
-  int numSteps = 0;
-  while (t + kRoundoff < t_end) {
-    numSteps++;
-    hipError_t e = launch(num, t, t_next, pr, y, codes, cnt, s);
-    if (!hip_ok(e, "k_radau_step launch")) return -1;
-    t = t_next;
-    t_next = fmin(t_end, t_start + (numSteps + 1) * step);
-  }
-  return 0;
+  if (!(t_start + kRoundoff < t_end) || num <= 0) return 0;
+  hipError_t e = launch(num, t_start, t_end, step, pr, y, codes, cnt, s);
+  return hip_ok(e, "k_radau launch") ? 0 : -1;
 }
 
 bool ensure_init() {

@@ -21,6 +21,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string.h>
+
 #include <string>
 #include <vector>
 
@@ -32,6 +34,7 @@ constexpr int kKinMaxReactions = 4096;
 constexpr double kKinR = 8.31446261815324;
 constexpr double kKinPatm = 101325.0;
 constexpr double kKinLnTref = 6.907755278982137;  // ln(1000)
+constexpr double kKinLnC0R = -4.407418589020958;   // ln(1e-6 * 101325 / R): ln C0 = this - ln T
 // Guards that keep f finite for any finite state: Newton trial states of a
 // violent transient can carry T <= 0 or T ~ 1e7 K (ln T undefined, exp
 // overflow) or mass fractions whose weighted sums vanish, and the Radau error
@@ -42,25 +45,18 @@ constexpr double kKinTmax = 1.0e4;
 constexpr double kKinSmin = 1.0e-3;
 constexpr double kKinCpmin = 1.0;
 
+// Device view of a network: one packed blob (built by KineticsHost) that every
+// block copies into the front of its LDS at kernel start.  Byte offsets:
+//   [off_invW] double invW[ns]      [off_W] double W[ns]
+//   [off_th]   double thermo[5 ns]  [off_rc] double rc[3 nr] (lnA, beta, EaR)
+//   [off_sl]   uint64 slots[nr]     byte a = species of slot a, 0xff = empty
+//   [off_co]   int csr_off[ns + 1]  [off_ce] int csr[nnz] = (reaction << 8) | (nu & 0xff)
 struct KinParams {
-  int nr;
-  const double* W;       // [ns]
-  const double* invW;    // [ns]
-  const double* thermo;  // [ns*5]
-  const int* slots;      // [nr*6]
-  const double* rc;      // [nr*3] lnA, beta, EaR
-  const int* csr_off;    // [ns+1] species -> (reaction, nu) list
-  const int* csr_rxn;
-  const int* csr_nu;
+  const double* blob;  // device copy
+  int blob_bytes;      // multiple of 16
+  int ns, nr, nnz;
+  int off_invW, off_W, off_th, off_rc, off_sl, off_co, off_ce;
 };
-
-// Make LDS writes by some lanes of a wave visible to reads by other lanes of
-// the same wave (groups never span waves).
-__device__ __forceinline__ void group_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 template <int NTOT>
 struct MechKinetics {
@@ -70,14 +66,34 @@ struct MechKinetics {
   static constexpr bool kJacInLds = true;
 
   // per group: C, gRT, hR, cpR, col [NS each]; kf, kr, q, qT [nr each];
-  // Jc [NS*NS]; parked Jacobian [N*N]
+  // scratch [N*N] (dwdC rows while the Jacobian is assembled, the LU
+  // transposition while the iteration matrices are factored); parked
+  // Jacobian [N*N]
   __host__ __device__ static size_t group_doubles(int nr) {
-    return (size_t)5 * NS + (size_t)4 * nr + (size_t)NS * NS + (size_t)N * N;
+    return (size_t)5 * NS + (size_t)4 * nr + (size_t)2 * N * N;
   }
-  static size_t lds_bytes(const Params& p, int groups) { return groups * group_doubles(p.nr) * sizeof(double); }
+  static size_t lds_bytes(const Params& p, int groups) {
+    return (size_t)p.blob_bytes + groups * group_doubles(p.nr) * sizeof(double);
+  }
+
+  // Whole block: copy the network tables into LDS (before any divergence).
+  __device__ __forceinline__ static void block_init(const Params& p, double* lds) {
+    const int n16 = p.blob_bytes / 16;
+    const int4* src = reinterpret_cast<const int4*>(p.blob);
+    int4* dst = reinterpret_cast<int4*>(lds);
+    for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+  }
 
   struct Ctx {
-    Params p;
+    int nr;
+    const double* invW;
+    const double* W;
+    const double* thermo;
+    const double* rc;
+    const unsigned long long* slots;
+    const int* csr_off;
+    const int* csr;
     double* C;
     double* gRT;
     double* hR;
@@ -88,9 +104,18 @@ struct MechKinetics {
     double* q;
     double* qT;
     double* Jc;
+    double* scratch;
     double* jac;
-    __device__ Ctx(const Params& pp, int g, double* lds) : p(pp) {
-      double* b = lds + (size_t)g * group_doubles(pp.nr);
+    __device__ Ctx(const Params& pp, int g, double* lds) : nr(pp.nr) {
+      const char* t = reinterpret_cast<const char*>(lds);
+      invW = reinterpret_cast<const double*>(t + pp.off_invW);
+      W = reinterpret_cast<const double*>(t + pp.off_W);
+      thermo = reinterpret_cast<const double*>(t + pp.off_th);
+      rc = reinterpret_cast<const double*>(t + pp.off_rc);
+      slots = reinterpret_cast<const unsigned long long*>(t + pp.off_sl);
+      csr_off = reinterpret_cast<const int*>(t + pp.off_co);
+      csr = reinterpret_cast<const int*>(t + pp.off_ce);
+      double* b = lds + pp.blob_bytes / 8 + (size_t)g * group_doubles(pp.nr);
       C = b; b += NS;
       gRT = b; b += NS;
       hR = b; b += NS;
@@ -100,50 +125,59 @@ struct MechKinetics {
       kr = b; b += pp.nr;
       q = b; b += pp.nr;
       qT = b; b += pp.nr;
-      Jc = b; b += NS * NS;
+      Jc = b;
+      scratch = b;
+      b += N * N;
       jac = b;
     }
+  };
+
+  struct Mix {
+    double T, lnT, iT, S, Ctot, cpm, dcpm;
   };
 
   // Shared prologue: T, thermo of owned species (to LDS), mixture sums and
   // concentrations (to LDS).
   template <int G, int R>
-  __device__ __forceinline__ static void prologue(Ctx& c, double P, const double (&y)[R], double& T,
-                                                  double& S, double& Ctot, double& cpm,
-                                                  double& dcpm) {
+  __device__ __forceinline__ static void prologue(Ctx& c, double P, const double (&y)[R], Mix& m) {
     const int ln = Grp<G>::lane();
-    T = fmin(fmax(Grp<G>::bcast(y[0], 0), kKinTmin), kKinTmax);
+    const double T = fmin(fmax(Grp<G>::bcast(y[0], 0), kKinTmin), kKinTmax);
     const double lnT = log(T);
+    const double iT = 1.0 / T;
     double Sp = 0.0, cpp = 0.0, dcp = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int r = ln + G * q;
       if (r >= 1 && r < N) {
         const int s = r - 1;
-        const double* a = c.p.thermo + 5 * s;
+        const double* a = c.thermo + 5 * s;
         const double a0 = a[0], a1 = a[1], a2 = a[2], bh = a[3], bs = a[4];
         const double cp = a0 + a1 * T + a2 * T * T;
-        const double h = a0 * T + a1 * T * T / 2 + a2 * T * T * T / 3 + bh;
-        const double sR = a0 * lnT + a1 * T + a2 * T * T / 2 + bs;
+        const double h = a0 * T + a1 * T * T * 0.5 + a2 * T * T * T * (1.0 / 3.0) + bh;
+        const double sR = a0 * lnT + a1 * T + a2 * T * T * 0.5 + bs;
         c.cpR[s] = cp;
         c.hR[s] = h;
-        c.gRT[s] = h / T - sR;
-        const double iw = c.p.invW[s];
+        c.gRT[s] = h * iT - sR;
+        const double iw = c.invW[s];
         Sp += y[q] * iw;
         cpp += y[q] * cp * iw;
         dcp += y[q] * (a1 + 2 * a2 * T) * iw;
       }
     }
-    S = fmax(Grp<G>::sum(Sp), kKinSmin);
-    cpm = fmax(kKinR * Grp<G>::sum(cpp), kKinCpmin);
-    dcpm = kKinR * Grp<G>::sum(dcp);
-    Ctot = 1e-6 * P / (kKinR * T);
+    m.T = T;
+    m.lnT = lnT;
+    m.iT = iT;
+    m.S = fmax(Grp<G>::sum(Sp), kKinSmin);
+    m.cpm = fmax(kKinR * Grp<G>::sum(cpp), kKinCpmin);
+    m.dcpm = kKinR * Grp<G>::sum(dcp);
+    m.Ctot = (1e-6 / kKinR) * P * iT;
+    const double cs = m.Ctot / m.S;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int r = ln + G * q;
       if (r >= 1 && r < N) {
         const int s = r - 1;
-        c.C[s] = Ctot * (y[q] * c.p.invW[s]) / S;
+        c.C[s] = cs * (y[q] * c.invW[s]);
       }
     }
     group_lds_sync();
@@ -151,30 +185,36 @@ struct MechKinetics {
 
   // Rate constants and rates of progress of the reactions this lane handles.
   template <int G, bool JAC>
-  __device__ __forceinline__ static void reactions(Ctx& c, double T) {
+  __device__ __forceinline__ static void reactions(Ctx& c, const Mix& m) {
     const int ln = Grp<G>::lane();
-    const double lnT = log(T);
-    const double lnC0 = log(1e-6 * kKinPatm / (kKinR * T));
-    const double iT = 1.0 / T;
-    for (int j = ln; j < c.p.nr; j += G) {
-      const int* sl = c.p.slots + 6 * j;
-      const double lnA = c.p.rc[3 * j], beta = c.p.rc[3 * j + 1], EaR = c.p.rc[3 * j + 2];
+    const double lnC0 = kKinLnC0R - m.lnT;  // ln(1e-6 P_atm / (R T))
+    const double lnTr = m.lnT - kKinLnTref;
+    const double iT = m.iT;
+#pragma unroll 1
+    for (int j = ln; j < c.nr; j += G) {
+      const unsigned long long sw = c.slots[j];
+      const double lnA = c.rc[3 * j], beta = c.rc[3 * j + 1], EaR = c.rc[3 * j + 2];
       double sum_nu_g = 0.0, sum_nu_h = 0.0;
       int dnu = 0;
       double pf = 1.0, pr = 1.0;
 #pragma unroll
       for (int a = 0; a < 6; ++a) {
-        const int s = sl[a];
-        if (s >= 0) {
-          const double sg = a < 3 ? -1.0 : 1.0;
-          sum_nu_g += sg * c.gRT[s];
-          if (JAC) sum_nu_h += sg * c.hR[s];
-          dnu += a < 3 ? -1 : 1;
-          if (a < 3) pf *= c.C[s];
-          else pr *= c.C[s];
+        const int s = (int)((sw >> (8 * a)) & 0xffu);
+        if (s != 0xff) {
+          if (a < 3) {
+            sum_nu_g -= c.gRT[s];
+            if (JAC) sum_nu_h -= c.hR[s];
+            dnu -= 1;
+            pf *= c.C[s];
+          } else {
+            sum_nu_g += c.gRT[s];
+            if (JAC) sum_nu_h += c.hR[s];
+            dnu += 1;
+            pr *= c.C[s];
+          }
         }
       }
-      const double lnkf = lnA + beta * (lnT - kKinLnTref) - EaR * iT;
+      const double lnkf = lnA + beta * lnTr - EaR * iT;
       const double lnKc = -sum_nu_g + dnu * lnC0;
       const double kf = exp(lnkf);
       const double kr = exp(lnkf - lnKc);
@@ -182,8 +222,8 @@ struct MechKinetics {
       pr *= kr;
       c.q[j] = pf - pr;
       if (JAC) {
-        const double dlnkf = beta * iT + EaR * iT * iT;
-        const double dlnkr = dlnkf - (sum_nu_h * iT * iT - dnu * iT);
+        const double dlnkf = (beta + EaR * iT) * iT;
+        const double dlnkr = dlnkf - (sum_nu_h * iT - dnu) * iT;
         c.qT[j] = pf * dlnkf - pr * dlnkr;
         c.kf[j] = kf;
         c.kr[j] = kr;
@@ -193,12 +233,13 @@ struct MechKinetics {
   }
 
   template <int G, int R>
-  __device__ static void dydt(Ctx& c, double, double P, const double (&y)[R], double (&f)[R]) {
+  __device__ __forceinline__ static void dydt(Ctx& c, double, double P, const double (&y)[R],
+                                              double (&f)[R]) {
     const int ln = Grp<G>::lane();
-    double T, S, Ctot, cpm, dcpm;
-    prologue<G, R>(c, P, y, T, S, Ctot, cpm, dcpm);
-    reactions<G, false>(c, T);
-    const double D = S / Ctot;
+    Mix m;
+    prologue<G, R>(c, P, y, m);
+    reactions<G, false>(c, m);
+    const double D = m.S / m.Ctot;
     double Qp = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -207,27 +248,32 @@ struct MechKinetics {
       if (r >= 1 && r < N) {
         const int s = r - 1;
         double w = 0.0;
-        const int e1 = c.p.csr_off[s + 1];
-        for (int e = c.p.csr_off[s]; e < e1; ++e) w += (double)c.p.csr_nu[e] * c.q[c.p.csr_rxn[e]];
-        f[q] = c.p.W[s] * w * D;
+        const int e1 = c.csr_off[s + 1];
+#pragma unroll 2
+        for (int e = c.csr_off[s]; e < e1; ++e) {
+          const int ce = c.csr[e];
+          w += (double)(signed char)(ce & 0xff) * c.q[ce >> 8];
+        }
+        f[q] = c.W[s] * w * D;
         Qp += c.hR[s] * w;
       }
     }
     const double Q = kKinR * Grp<G>::sum(Qp);
 #pragma unroll
     for (int q = 0; q < R; ++q)
-      if (ln + G * q == 0) f[q] = -Q * D / cpm;
+      if (ln + G * q == 0) f[q] = -Q * D / m.cpm;
     group_lds_sync();  // LDS is reused by the next evaluation
   }
 
   template <int G, int R>
-  __device__ static void jacob(Ctx& c, double, double P, const double (&y)[R], double (&J)[R][N]) {
+  __device__ __forceinline__ static void jacob(Ctx& c, double, double P, const double (&y)[R],
+                                               double (&J)[R][N]) {
     const int ln = Grp<G>::lane();
-    double T, S, Ctot, cpm, dcpm;
-    prologue<G, R>(c, P, y, T, S, Ctot, cpm, dcpm);
-    reactions<G, true>(c, T);
+    Mix m;
+    prologue<G, R>(c, P, y, m);
+    reactions<G, true>(c, m);
+    const double S = m.S, Ctot = m.Ctot, iT = m.iT;
     const double D = S / Ctot;
-    const double iT = 1.0 / T;
     double Qp = 0.0, dQp = 0.0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -238,23 +284,26 @@ struct MechKinetics {
 #pragma unroll
         for (int i = 0; i < NS; ++i) row[i] = 0.0;
         double w = 0.0, wT = 0.0;
-        const int e1 = c.p.csr_off[s + 1];
-        for (int e = c.p.csr_off[s]; e < e1; ++e) {
-          const int j = c.p.csr_rxn[e];
-          const double nu = (double)c.p.csr_nu[e];
+        const int e1 = c.csr_off[s + 1];
+#pragma unroll 1
+        for (int e = c.csr_off[s]; e < e1; ++e) {
+          const int ce = c.csr[e];
+          const int j = ce >> 8;
+          const double nu = (double)(signed char)(ce & 0xff);
           w += nu * c.q[j];
           wT += nu * c.qT[j];
-          const int* sl = c.p.slots + 6 * j;
+          const unsigned long long sw = c.slots[j];
           int sp[6];
-#pragma unroll
-          for (int a = 0; a < 6; ++a) sp[a] = sl[a];
           double cv[6];
 #pragma unroll
-          for (int a = 0; a < 6; ++a) cv[a] = sp[a] >= 0 ? c.C[sp[a]] : 1.0;
+          for (int a = 0; a < 6; ++a) {
+            sp[a] = (int)((sw >> (8 * a)) & 0xffu);
+            cv[a] = sp[a] != 0xff ? c.C[sp[a]] : 1.0;
+          }
           const double kfj = nu * c.kf[j], krj = -nu * c.kr[j];
 #pragma unroll
           for (int b = 0; b < 6; ++b) {
-            if (sp[b] >= 0) {
+            if (sp[b] != 0xff) {
               double v = b < 3 ? kfj : krj;
               const int lo = b < 3 ? 0 : 3;
 #pragma unroll
@@ -272,10 +321,10 @@ struct MechKinetics {
           jr[i] = row[i];
           rs += jr[i] * c.C[i];
         }
-        const double Ws = c.p.W[s];
+        const double Ws = c.W[s];
         const double corr = (rs - w) / Ctot;
 #pragma unroll
-        for (int i = 0; i < NS; ++i) J[q][1 + i] = Ws * c.p.invW[i] * (jr[i] - corr);
+        for (int i = 0; i < NS; ++i) J[q][1 + i] = Ws * c.invW[i] * (jr[i] - corr);
         J[q][0] = Ws * D * (wT + (w - rs) * iT);
         const double hs = c.hR[s];
 #pragma unroll
@@ -293,23 +342,25 @@ struct MechKinetics {
       const int r = ln + G * q;
       if (r >= 1 && r < N) {
         const int i = r - 1;
-        double s = 0.0;
-        for (int k = 0; k < NS; ++k) s += c.Jc[(size_t)k * NS + i];
-        c.col[i] = s;
+        double sum = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < NS; ++k) sum += c.Jc[(size_t)k * NS + i];
+        c.col[i] = sum;
       }
     }
     group_lds_sync();
+    const double cpm = m.cpm;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       if (ln + G * q == 0) {
         const double QD = Q * D;
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-          const double iw = c.p.invW[i];
+          const double iw = c.invW[i];
           const double dQdYi = kKinR * c.col[i] * iw / S;
           J[q][1 + i] = -(D * dQdYi + Q * iw / Ctot) / cpm + QD * kKinR * c.cpR[i] * iw / (cpm * cpm);
         }
-        J[q][0] = -(D * dQdT + QD * iT) / cpm + QD * dcpm / (cpm * cpm);
+        J[q][0] = -(D * dQdT + QD * iT) / cpm + QD * m.dcpm / (cpm * cpm);
       } else if (ln + G * q >= N) {
 #pragma unroll
         for (int i = 0; i < N; ++i) J[q][i] = 0.0;
@@ -396,31 +447,48 @@ struct KineticsHost {
 
   bool upload(std::string* msg) {
     release();
-    const size_t nW = ns, nT = 5 * (size_t)ns, nS = 6 * (size_t)nr, nR = 3 * (size_t)nr;
-    const size_t nO = ns + 1, nE = csr_rxn.size() ? csr_rxn.size() : 1;
-    const size_t bytes = (2 * nW + nT + nR) * sizeof(double) + (nS + nO + 2 * nE) * sizeof(int) + 256;
-    if (hipMalloc(&dmem, bytes) != hipSuccess) {
+    // packed blob, every section 16-byte aligned (see KinParams)
+    const int nnz = (int)csr_rxn.size();
+    std::vector<char> blob;
+    auto section = [&](const void* src, size_t bytes) {
+      const int off = (int)blob.size();
+      blob.resize(blob.size() + ((bytes + 15) / 16) * 16, 0);
+      if (bytes) memcpy(blob.data() + off, src, bytes);
+      return off;
+    };
+    std::vector<unsigned long long> sl(nr);
+    for (int j = 0; j < nr; ++j) {
+      unsigned long long w = 0;
+      for (int a = 0; a < 6; ++a) {
+        const int sp = slots[6 * j + a];
+        w |= (unsigned long long)(sp < 0 ? 0xff : sp) << (8 * a);
+      }
+      sl[j] = w;
+    }
+    std::vector<int> ce(nnz > 0 ? nnz : 1, 0);
+    for (int e = 0; e < nnz; ++e) ce[e] = (csr_rxn[e] << 8) | (csr_nu[e] & 0xff);
+    dev.ns = ns;
+    dev.nr = nr;
+    dev.nnz = nnz;
+    dev.off_invW = section(invW.data(), ns * sizeof(double));
+    dev.off_W = section(W.data(), ns * sizeof(double));
+    dev.off_th = section(thermo.data(), 5 * (size_t)ns * sizeof(double));
+    dev.off_rc = section(rc.data(), 3 * (size_t)nr * sizeof(double));
+    dev.off_sl = section(sl.data(), (size_t)nr * sizeof(unsigned long long));
+    dev.off_co = section(csr_off.data(), (size_t)(ns + 1) * sizeof(int));
+    dev.off_ce = section(ce.data(), (size_t)ce.size() * sizeof(int));
+    dev.blob_bytes = (int)blob.size();
+    if (hipMalloc(&dmem, blob.size()) != hipSuccess) {
       *msg = "hipMalloc failed for kinetics tables";
       dmem = nullptr;
       return false;
     }
-    char* p = (char*)dmem;
-    auto put = [&](const void* src, size_t n, size_t esz) {
-      void* dst = p;
-      if (n) hipMemcpy(dst, src, n * esz, hipMemcpyHostToDevice);
-      p += ((n * esz + 15) / 16) * 16;
-      return dst;
-    };
-    dev.nr = nr;
-    dev.W = (const double*)put(W.data(), nW, sizeof(double));
-    dev.invW = (const double*)put(invW.data(), nW, sizeof(double));
-    dev.thermo = (const double*)put(thermo.data(), nT, sizeof(double));
-    dev.rc = (const double*)put(rc.data(), nR, sizeof(double));
-    dev.slots = (const int*)put(slots.data(), nS, sizeof(int));
-    dev.csr_off = (const int*)put(csr_off.data(), nO, sizeof(int));
-    dev.csr_rxn = (const int*)put(csr_rxn.data(), csr_rxn.size(), sizeof(int));
-    dev.csr_nu = (const int*)put(csr_nu.data(), csr_nu.size(), sizeof(int));
-    return hipDeviceSynchronize() == hipSuccess;
+    if (hipMemcpy(dmem, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      *msg = "hipMemcpy failed for kinetics tables";
+      return false;
+    }
+    dev.blob = (const double*)dmem;
+    return true;
   }
 
   template <int NTOT>

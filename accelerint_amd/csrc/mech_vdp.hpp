@@ -14,8 +14,10 @@ struct MechVdP {
   static constexpr bool kJacInLds = false;
   struct Params {};
   static size_t lds_bytes(const Params&, int) { return 0; }
+  __device__ __forceinline__ static void block_init(const Params&, double*) {}
   struct Ctx {
     double* jac = nullptr;
+    double* scratch = nullptr;
     __device__ Ctx(const Params&, int /*group*/, double* /*lds*/) {}
   };
 

@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace acc {
 
 // ---- control constants: radau2a.c:37-66 ------------------------------------
@@ -130,6 +132,14 @@ struct Grp {
       return __shfl(v, src, G);
     }
   }
+  // per-lane source (ds_bpermute)
+  __device__ __forceinline__ static double gather(double v, int src) {
+    if constexpr (G == 1) {
+      return v;
+    } else {
+      return __shfl(v, src, G);
+    }
+  }
   __device__ __forceinline__ static double xor_(double v, int m) { return __shfl_xor(v, m, G); }
   __device__ __forceinline__ static int xor_i(int v, int m) { return __shfl_xor(v, m, G); }
 
@@ -202,6 +212,11 @@ __device__ __forceinline__ void group_argmax(double& bv, int& bkey) {
 // (columns >= pos[q]); pos[q] is the elimination step at which row r was the
 // pivot; perm.get(k) is the pivot row of step k; rd[q] = 1 / U(r, pos[q]).
 // info = first step (1-based) with an exactly zero pivot, 0 otherwise.
+//
+// Rows that do not take part in step k (the pivot row, rows already
+// eliminated, padding rows) get a zero multiplier, so the update is
+// branch-free and each pivot-row entry is broadcast just before the column
+// update that consumes it.
 template <int N, int G, int R>
 __device__ __forceinline__ void lu_real(double (&a)[R][N], int (&pos)[R], Perm<N>& perm,
                                         double (&rd)[R], int& info) {
@@ -240,34 +255,29 @@ __device__ __forceinline__ void lu_real(double (&a)[R][N], int (&pos)[R], Perm<N
       if (pos[q] == k) pos[q] = ppos;  // the row LAPACK swaps out of position k
       if (Rows<N, G, R>::row(ln, q) == prow) pos[q] = k;
     }
-    if (piv != 0.0) {
-      const bool tiny = fabs(piv) < kDblMin;
-      const double rp = 1.0 / piv;
+    const bool nz = piv != 0.0;
+    if (!nz && info == 0) info = k + 1;
+    const bool tiny = fabs(piv) < kDblMin;
+    const double rp = 1.0 / piv;
+    double l[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q)
-        if (Rows<N, G, R>::row(ln, q) == prow) rd[q] = rp;
-      double u[N];
+    for (int q = 0; q < R; ++q) {
+      if (Rows<N, G, R>::row(ln, q) == prow) rd[q] = nz ? rp : 0.0;
+      const bool act = nz && pos[q] > k;
+      const double lq = tiny ? a[q][k] / piv : a[q][k] * rp;
+      l[q] = act ? lq : 0.0;
+      if (act) a[q][k] = lq;
+    }
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        if (j > k) {
-          double m2[R];
+    for (int j = 0; j < N; ++j) {
+      if (j > k) {
+        double m2[R];
 #pragma unroll
-          for (int q = 0; q < R; ++q) m2[q] = a[q][j];
-          u[j] = Grp<G>::bcast(sel<R>(m2, pq), plane);
-        }
+        for (int q = 0; q < R; ++q) m2[q] = a[q][j];
+        const double uj = Grp<G>::bcast(sel<R>(m2, pq), plane);
+#pragma unroll
+        for (int q = 0; q < R; ++q) a[q][j] = fma(-l[q], uj, a[q][j]);
       }
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        if (pos[q] > k) {
-          const double l = tiny ? a[q][k] / piv : a[q][k] * rp;
-          a[q][k] = l;
-#pragma unroll
-          for (int j = 0; j < N; ++j)
-            if (j > k) a[q][j] = fma(-l, u[j], a[q][j]);
-        }
-      }
-    } else if (info == 0) {
-      info = k + 1;
     }
   }
 }
@@ -314,52 +324,50 @@ __device__ __forceinline__ void lu_cplx(double (&ar)[R][N], double (&ai)[R][N], 
       if (pos[q] == k) pos[q] = ppos;
       if (Rows<N, G, R>::row(ln, q) == prow) pos[q] = k;
     }
-    if (pr_ != 0.0 || pi_ != 0.0) {
-      double rr, ri;
-      crecip(pr_, pi_, rr, ri);
+    const bool nz = (pr_ != 0.0 || pi_ != 0.0);
+    if (!nz && info == 0) info = k + 1;
+    double rr, ri;
+    crecip(pr_, pi_, rr, ri);
+    double lr[R], li[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q)
-        if (Rows<N, G, R>::row(ln, q) == prow) { rdr[q] = rr; rdi[q] = ri; }
-      double ur[N], ui[N];
+    for (int q = 0; q < R; ++q) {
+      if (Rows<N, G, R>::row(ln, q) == prow) {
+        rdr[q] = nz ? rr : 0.0;
+        rdi[q] = nz ? ri : 0.0;
+      }
+      const bool act = nz && pos[q] > k;
+      const double xr = ar[q][k], xi = ai[q][k];
+      const double vr = xr * rr - xi * ri;
+      const double vi = xr * ri + xi * rr;
+      lr[q] = act ? vr : 0.0;
+      li[q] = act ? vi : 0.0;
+      if (act) { ar[q][k] = vr; ai[q][k] = vi; }
+    }
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        if (j > k) {
-          double m2r[R], m2i[R];
+    for (int j = 0; j < N; ++j) {
+      if (j > k) {
+        double m2r[R], m2i[R];
 #pragma unroll
-          for (int q = 0; q < R; ++q) { m2r[q] = ar[q][j]; m2i[q] = ai[q][j]; }
-          ur[j] = Grp<G>::bcast(sel<R>(m2r, pq), plane);
-          ui[j] = Grp<G>::bcast(sel<R>(m2i, pq), plane);
+        for (int q = 0; q < R; ++q) { m2r[q] = ar[q][j]; m2i[q] = ai[q][j]; }
+        const double ur = Grp<G>::bcast(sel<R>(m2r, pq), plane);
+        const double ui = Grp<G>::bcast(sel<R>(m2i, pq), plane);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          ar[q][j] = fma(-lr[q], ur, ar[q][j]);
+          ar[q][j] = fma(li[q], ui, ar[q][j]);
+          ai[q][j] = fma(-lr[q], ui, ai[q][j]);
+          ai[q][j] = fma(-li[q], ur, ai[q][j]);
         }
       }
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        if (pos[q] > k) {
-          const double xr = ar[q][k], xi = ai[q][k];
-          const double lr = xr * rr - xi * ri;
-          const double li = xr * ri + xi * rr;
-          ar[q][k] = lr;
-          ai[q][k] = li;
-#pragma unroll
-          for (int j = 0; j < N; ++j) {
-            if (j > k) {
-              ar[q][j] = fma(-lr, ur[j], ar[q][j]);
-              ar[q][j] = fma(li, ui[j], ar[q][j]);
-              ai[q][j] = fma(-lr, ui[j], ai[q][j]);
-              ai[q][j] = fma(-li, ur[j], ai[q][j]);
-            }
-          }
-        }
-      }
-    } else if (info == 0) {
-      info = k + 1;
     }
   }
 }
 
 // ---- triangular solves (dgetrs / zgetrs, nrhs = 1) ---------------------------
-// b[q] is the right-hand side of row r = row(ln, q); on return x[q] holds
+// b[q] is the right-hand side of row r = row(ln, q); on return it holds
 // solution component r.  Forward then backward column-oriented substitution in
-// pivot order, as reference BLAS dtrsm does it.
+// pivot order, as reference BLAS dtrsm does it; rows outside the current
+// step's range take a zero coefficient.
 template <int N, int G, int R>
 __device__ __forceinline__ void solve_real(const double (&a)[R][N], const int (&pos)[R],
                                            const Perm<N>& perm, const double (&rd)[R],
@@ -370,8 +378,7 @@ __device__ __forceinline__ void solve_real(const double (&a)[R][N], const int (&
     const int prow = perm.get(k);
     const double yk = Grp<G>::bcast(sel<R>(b, prow / G), prow % G);
 #pragma unroll
-    for (int q = 0; q < R; ++q)
-      if (pos[q] > k) b[q] = fma(-a[q][k], yk, b[q]);
+    for (int q = 0; q < R; ++q) b[q] = fma(-(pos[q] > k ? a[q][k] : 0.0), yk, b[q]);
   }
   double x[R];
 #pragma unroll
@@ -383,7 +390,7 @@ __device__ __forceinline__ void solve_real(const double (&a)[R][N], const int (&
     const double xk = Grp<G>::bcast(sel<R>(b, pq) * sel<R>(rd, pq), prow % G);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (pos[q] >= 0 && pos[q] < k) b[q] = fma(-a[q][k], xk, b[q]);
+      b[q] = fma(-((pos[q] >= 0 && pos[q] < k) ? a[q][k] : 0.0), xk, b[q]);
       if (Rows<N, G, R>::row(ln, q) == k) x[q] = xk;
     }
   }
@@ -405,13 +412,12 @@ __device__ __forceinline__ void solve_cplx(const double (&ar)[R][N], const doubl
     const double yi = Grp<G>::bcast(sel<R>(bi, pq), pl);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (pos[q] > k) {
-        const double lr = ar[q][k], li = ai[q][k];
-        br[q] = fma(-lr, yr, br[q]);
-        br[q] = fma(li, yi, br[q]);
-        bi[q] = fma(-lr, yi, bi[q]);
-        bi[q] = fma(-li, yr, bi[q]);
-      }
+      const bool act = pos[q] > k;
+      const double lr = act ? ar[q][k] : 0.0, li = act ? ai[q][k] : 0.0;
+      br[q] = fma(-lr, yr, br[q]);
+      br[q] = fma(li, yi, br[q]);
+      bi[q] = fma(-lr, yi, bi[q]);
+      bi[q] = fma(-li, yr, bi[q]);
     }
   }
   double xr[R], xi[R];
@@ -427,18 +433,301 @@ __device__ __forceinline__ void solve_cplx(const double (&ar)[R][N], const doubl
     const double ki = Grp<G>::bcast(vr * di + vi * dr, pl);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (pos[q] >= 0 && pos[q] < k) {
-        const double ur = ar[q][k], ui = ai[q][k];
-        br[q] = fma(-ur, kr, br[q]);
-        br[q] = fma(ui, ki, br[q]);
-        bi[q] = fma(-ur, ki, bi[q]);
-        bi[q] = fma(-ui, kr, bi[q]);
-      }
+      const bool act = pos[q] >= 0 && pos[q] < k;
+      const double ur = act ? ar[q][k] : 0.0, ui = act ? ai[q][k] : 0.0;
+      br[q] = fma(-ur, kr, br[q]);
+      br[q] = fma(ui, ki, br[q]);
+      bi[q] = fma(-ur, ki, bi[q]);
+      bi[q] = fma(-ui, kr, bi[q]);
       if (Rows<N, G, R>::row(ln, q) == k) { xr[q] = kr; xi[q] = ki; }
     }
   }
 #pragma unroll
   for (int q = 0; q < R; ++q) { br[q] = xr[q]; bi[q] = xi[q]; }
+}
+
+// Orders LDS traffic between lanes of one wavefront (a group never spans
+// waves): LDS operations of a wave complete in issue order, so a compiler
+// barrier plus an LDS-counter wait suffices; no global-memory wait is implied.
+__device__ __forceinline__ void group_lds_sync() {
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt/expcnt untouched
+  __builtin_amdgcn_wave_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+// ---- compile-time-lane primitives -------------------------------------------
+// Loops whose index must be a constant expression (a DPP control or a readlane
+// lane) are written with static_for.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Broadcast the value held by lane K of each lane group to the whole group.
+// G = 16: DPP row_newbcast (one VALU move per dword, no LDS round trip);
+// G = 64: readlane into SGPRs; other widths fall back to ds_bpermute.
+template <int G, int K>
+__device__ __forceinline__ int bcast_k_i(int v) {
+  if constexpr (G == 1) {
+    return v;
+  } else if constexpr (G == 16) {
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + K, 0xf, 0xf, false);
+  } else if constexpr (G == 64) {
+    return __builtin_amdgcn_readlane(v, K);
+  } else {
+    return __shfl(v, K, G);
+  }
+}
+template <int G, int K>
+__device__ __forceinline__ double bcast_k(double v) {
+  if constexpr (G == 1) {
+    return v;
+  } else {
+    const int lo = bcast_k_i<G, K>(__double2loint(v));
+    const int hi = bcast_k_i<G, K>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+  }
+}
+
+// Packed original-row table for the explicit-interchange LU: byte i = the
+// original row now in position i.
+template <int N>
+struct RowMap {
+  uint32_t w[(N + 3) / 4];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int i = 0; i < (N + 3) / 4; ++i) {
+      uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) v |= (uint32_t)((4 * i + b) & 0xff) << (8 * b);
+      w[i] = v;
+    }
+  }
+  __device__ __forceinline__ int get(int i) const {  // i may be a runtime value
+    uint32_t word = w[0];
+#pragma unroll
+    for (int q = 1; q < (N + 3) / 4; ++q) word = ((i >> 2) == q) ? w[q] : word;
+    return (int)((word >> (8 * (i & 3))) & 0xffu);
+  }
+  __device__ __forceinline__ void set(int i, int v) {  // runtime i
+    const int s = 8 * (i & 3);
+#pragma unroll
+    for (int q = 0; q < (N + 3) / 4; ++q)
+      if ((i >> 2) == q) w[q] = (w[q] & ~(0xffu << s)) | ((uint32_t)v << s);
+  }
+  __device__ __forceinline__ void swap(int i, int j) {
+    const int vi = get(i), vj = get(j);
+    set(i, vj);
+    set(j, vi);
+  }
+};
+
+// ---- column-layout LU (dgetf2 / zgetf2 with explicit interchanges) --------
+// Lane j of the group holds column j: a[i] = A(i, j).  For G >= N only.  The
+// pivot search of step k runs inside lane k (first maximum of |a| or
+// |Re|+|Im|, exactly LAPACK's idamax/izamax order since rows are physically
+// interchanged), the interchange is a register select in every lane, lane k
+// scales its column into multipliers and broadcasts them from a compile-time
+// lane.  On exit a[] holds LAPACK's combined L\U column, rmap the row
+// permutation, rdiag = 1/U(j, j) in lane j; info as LAPACK.
+template <int N, int G>
+__device__ __forceinline__ void clu_real(double (&a)[N], RowMap<N>& rmap, double& rdiag, int& info) {
+  static_assert(G >= N, "column LU needs a lane per column");
+  const int ln = Grp<G>::lane();
+  rmap.init();
+  info = 0;
+  rdiag = 0.0;
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    double vmax = fabs(a[k]);
+    int p = k;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double v = fabs(a[i]);
+      const bool gt = v > vmax;
+      vmax = gt ? v : vmax;
+      p = gt ? i : p;
+    }
+    const int code = bcast_k_i<G, k>(vmax != 0.0 ? p : (k | 0x100));
+    const bool nz = (code & 0x100) == 0;
+    p = code & 0xff;
+    if (!nz && info == 0) info = k + 1;
+    if (nz) rmap.swap(k, p);
+    // interchange rows k and p in every column
+    const double ak = a[k];
+    double ap = ak;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) ap = (i == p) ? a[i] : ap;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) a[i] = (i == p) ? ak : a[i];
+    a[k] = ap;
+    // lane k scales its column into multipliers (the other lanes scale by 1,
+    // which is exact).  LAPACK divides instead when |pivot| < DBL_MIN; that
+    // case only arises for subnormal pivots and is not reproduced.
+    const double rp = 1.0 / ap;
+    const bool mine = (ln == k) && nz;
+    if (ln == k) rdiag = nz ? rp : 0.0;
+    const double sf = mine ? rp : 1.0;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) a[i] *= sf;
+    // rank-1 update of the columns right of k
+    const double uk = (ln > k && ln < N) ? a[k] : 0.0;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double li = bcast_k<G, k>(a[i]);
+      a[i] = fma(-li, uk, a[i]);
+    }
+  });
+}
+
+template <int N, int G>
+__device__ __forceinline__ void clu_cplx(double (&ar)[N], double (&ai)[N], RowMap<N>& rmap,
+                                         double& rdr, double& rdi, int& info) {
+  static_assert(G >= N, "column LU needs a lane per column");
+  const int ln = Grp<G>::lane();
+  rmap.init();
+  info = 0;
+  rdr = 0.0;
+  rdi = 0.0;
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    double vmax = fabs(ar[k]) + fabs(ai[k]);
+    int p = k;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double v = fabs(ar[i]) + fabs(ai[i]);
+      const bool gt = v > vmax;
+      vmax = gt ? v : vmax;
+      p = gt ? i : p;
+    }
+    const int code = bcast_k_i<G, k>(vmax != 0.0 ? p : (k | 0x100));
+    const bool nz = (code & 0x100) == 0;
+    p = code & 0xff;
+    if (!nz && info == 0) info = k + 1;
+    if (nz) rmap.swap(k, p);
+    const double akr = ar[k], aki = ai[k];
+    double apr = akr, api = aki;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      apr = (i == p) ? ar[i] : apr;
+      api = (i == p) ? ai[i] : api;
+    }
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      ar[i] = (i == p) ? akr : ar[i];
+      ai[i] = (i == p) ? aki : ai[i];
+    }
+    ar[k] = apr;
+    ai[k] = api;
+    double rr, ri;
+    crecip(apr, api, rr, ri);
+    const bool mine = (ln == k) && nz;
+    if (ln == k) {
+      rdr = nz ? rr : 0.0;
+      rdi = nz ? ri : 0.0;
+    }
+    // lane k scales by the reciprocal pivot, the others by 1 + 0i (exact)
+    const double sr = mine ? rr : 1.0, si = mine ? ri : 0.0;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double xr = ar[i], xi = ai[i];
+      ar[i] = xr * sr - xi * si;
+      ai[i] = xr * si + xi * sr;
+    }
+    const bool upd = ln > k && ln < N;
+    const double ukr = upd ? ar[k] : 0.0, uki = upd ? ai[k] : 0.0;
+#pragma unroll
+    for (int i = k + 1; i < N; ++i) {
+      const double lr = bcast_k<G, k>(ar[i]);
+      const double li = bcast_k<G, k>(ai[i]);
+      ar[i] = fma(-lr, ukr, ar[i]);
+      ar[i] = fma(li, uki, ar[i]);
+      ai[i] = fma(-lr, uki, ai[i]);
+      ai[i] = fma(-li, ukr, ai[i]);
+    }
+  });
+}
+
+// Column layout (lane j holds column j) -> row layout (lane i holds row i) of
+// an N x N block through the group's LDS scratch (N*N doubles).
+template <int N, int G>
+__device__ __forceinline__ void transpose_cols_to_rows(const double (&col)[N], double (&row)[N],
+                                                       double* scratch) {
+  const int ln = Grp<G>::lane();
+  if (ln < N) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) scratch[i * N + ln] = col[i];
+  }
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_wave_barrier();
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int j = 0; j < N; ++j) row[j] = (ln < N) ? scratch[ln * N + j] : 0.0;
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_wave_barrier();
+  __asm__ volatile("" ::: "memory");
+}
+
+// ---- row-layout solves for the explicit-interchange factors ----------------
+// Lane i holds row i of L\U (LAPACK order) and src = the original row whose
+// equation sits in position i; b[] is the right-hand side of the lane's own
+// (original) equation.  Pivot rows are at compile-time lanes, so every
+// broadcast comes from a fixed lane.  On return b holds solution component i
+// in lane i.
+template <int N, int G>
+__device__ __forceinline__ void rsolve_real(const double (&a)[N], int src, double rdiag,
+                                            double& b) {
+  const int ln = Grp<G>::lane();
+  double x = Grp<G>::gather(b, src);  // (P b)_i
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const double yk = bcast_k<G, k>(x);
+    x = fma(-(ln > k && ln < N ? a[k] : 0.0), yk, x);
+  });
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = N - 1 - decltype(kc)::value;
+    const double xk = bcast_k<G, k>(x * rdiag);
+    if (ln == k) x = xk;
+    x = fma(-(ln < k ? a[k] : 0.0), xk, x);
+  });
+  b = (ln < N) ? x : 0.0;
+}
+
+template <int N, int G>
+__device__ __forceinline__ void rsolve_cplx(const double (&ar)[N], const double (&ai)[N], int src,
+                                            double rdr, double rdi, double& br, double& bi) {
+  const int ln = Grp<G>::lane();
+  double xr = Grp<G>::gather(br, src), xi = Grp<G>::gather(bi, src);
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const double yr = bcast_k<G, k>(xr), yi = bcast_k<G, k>(xi);
+    const bool act = ln > k && ln < N;
+    const double lr = act ? ar[k] : 0.0, li = act ? ai[k] : 0.0;
+    xr = fma(-lr, yr, xr);
+    xr = fma(li, yi, xr);
+    xi = fma(-lr, yi, xi);
+    xi = fma(-li, yr, xi);
+  });
+  static_for<0, N>([&](auto kc) {
+    constexpr int k = N - 1 - decltype(kc)::value;
+    const double kr = bcast_k<G, k>(xr * rdr - xi * rdi);
+    const double ki = bcast_k<G, k>(xr * rdi + xi * rdr);
+    if (ln == k) { xr = kr; xi = ki; }
+    const bool act = ln < k;
+    const double ur = act ? ar[k] : 0.0, ui = act ? ai[k] : 0.0;
+    xr = fma(-ur, kr, xr);
+    xr = fma(ui, ki, xr);
+    xi = fma(-ur, ki, xi);
+    xi = fma(-ui, kr, xi);
+  });
+  br = (ln < N) ? xr : 0.0;
+  bi = (ln < N) ? xi : 0.0;
 }
 
 // ---- the integrator ----------------------------------------------------------
@@ -451,14 +740,97 @@ __device__ __forceinline__ void solve_cplx(const double (&ar)[R][N], const doubl
 // When kJacInLds is set the Jacobian is parked in the group's LDS between
 // evaluations (it is only read when the iteration matrices are rebuilt),
 // which keeps N*R doubles per lane out of the register file.
+//
+// The reference's integrate() (radau2a.c:520-736) is a loop over step
+// attempts.  Here it is a resumable state machine: begin() is the prologue
+// (:521-564) and attempt() runs one pass of the loop body (:566-734) and
+// reports whether the interval is finished, so a persistent kernel can
+// interleave step attempts of many conditions and refill a lane group with a
+// new condition or a new global step as soon as its current one ends.
+enum : int { RADAU_CONTINUE = -1 };
+
 template <class Mech, int G, int R>
 struct Radau {
   static constexpr int N = Mech::N;
   using RW = Rows<N, G, R>;
   using Ctx = typename Mech::Ctx;
+  // Column-layout LU with compile-time broadcast lanes when one lane per
+  // column fits in a DPP row (G = 16) or a wave (G = 64) and the Jacobian is
+  // parked in LDS (it is read back column-wise); implicit-pivot row LU
+  // otherwise.
+  static constexpr bool kColLU = (G == 16 || G == 64) && G >= N && R == 1 && Mech::kJacInLds;
+
+  // per-interval controller state (group-uniform)
+  double t, t_end, H, Hold, Hacc, ErrOld, NewtonRate;
+  int Nconsecutive, Nsteps;
+  bool Reject, FirstStep, SkipJac, SkipLU;
+  // row-owned vectors
+  double y[R], sc[R], y0[R], F0[R], Z1[R], Z2[R], Z3[R];
+  double CONT0[R], CONT1[R], CONT2[R];
+  // factorizations
+  double E1[R][N], E2r[R][N], E2i[R][N];
+  int pos1[R], pos2[R];
+  double rd1[R], rd2r[R], rd2i[R];
+  Perm<N> perm1, perm2;
+  int src1, src2;  // column-LU path: original equation of this lane's row
+  double A[R][N];  // Jacobian (only live across attempts when !Mech::kJacInLds)
+
+  __device__ __forceinline__ void solve1(double (&b)[R]) const {
+    if constexpr (kColLU) rsolve_real<N, G>(E1[0], src1, rd1[0], b[0]);
+    else solve_real<N, G, R>(E1, pos1, perm1, rd1, b);
+  }
+  __device__ __forceinline__ void solve2(double (&br)[R], double (&bi)[R]) const {
+    if constexpr (kColLU) rsolve_cplx<N, G>(E2r[0], E2i[0], src2, rd2r[0], rd2i[0], br[0], bi[0]);
+    else solve_cplx<N, G, R>(E2r, E2i, pos2, perm2, rd2r, rd2i, br, bi);
+  }
+
+  // RK_Decomp, radau2a.c:256-278: E1 = gamma/H - J, E2 = (alpha + i beta)/H - J,
+  // then the real and (if E1 is regular) the complex LU.  Returns info.
+  __device__ __forceinline__ int decomp(Ctx& ctx) {
+    const int ln = Grp<G>::lane();
+    const double g1 = rkGamma / H, g2r = rkAlpha / H, g2i = rkBeta / H;
+    int info = 0;
+    if constexpr (kColLU) {
+      group_lds_sync();  // the parked Jacobian rows of all lanes
+      double c1[N], c2r[N], c2i[N];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double jij = (ln < N) ? ctx.jac[i * N + ln] : 0.0;
+        const bool d = (i == ln);
+        c1[i] = -jij + (d ? g1 : 0.0);
+        c2r[i] = -jij + (d ? g2r : 0.0);
+        c2i[i] = d ? g2i : 0.0;
+      }
+      RowMap<N> m1, m2;
+      clu_real<N, G>(c1, m1, rd1[0], info);
+      transpose_cols_to_rows<N, G>(c1, E1[0], ctx.scratch);
+      src1 = m1.get(ln);
+      if (info == 0) {
+        clu_cplx<N, G>(c2r, c2i, m2, rd2r[0], rd2i[0], info);
+        transpose_cols_to_rows<N, G>(c2r, E2r[0], ctx.scratch);
+        transpose_cols_to_rows<N, G>(c2i, E2i[0], ctx.scratch);
+        src2 = m2.get(ln);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = RW::row(ln, q);
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const bool d = (j == r);
+          E1[q][j] = -A[q][j] + (d ? g1 : 0.0);
+          E2r[q][j] = -A[q][j] + (d ? g2r : 0.0);
+          E2i[q][j] = d ? g2i : 0.0;
+        }
+      }
+      lu_real<N, G, R>(E1, pos1, perm1, rd1, info);
+      if (info == 0) lu_cplx<N, G, R>(E2r, E2i, pos2, perm2, rd2r, rd2i, info);
+    }
+    return info;
+  }
 
   // RK_ErrorNorm, radau2a.c:446-453: max(sqrt(sum (sc d)^2 / N), 1e-10)
-  __device__ __forceinline__ static double err_norm(const double (&sc)[R], const double (&d)[R]) {
+  __device__ __forceinline__ double err_norm(const double (&d)[R]) const {
     const int ln = Grp<G>::lane();
     double s = 0.0;
 #pragma unroll
@@ -468,300 +840,264 @@ struct Radau {
     return fmax(sqrt(s / (double)N), 1e-10);
   }
 
-  // integrate(), radau2a.c:520-736.  y[q] is the owned part of the state.
-  __device__ static int integrate(Ctx& ctx, const SolverParams& P, double t_start, double t_end,
-                                  double pr, double (&y)[R], uint32_t (&cnt)[NCNT]) {
-    const int ln = Grp<G>::lane();
-    const double atol = P.atol, rtol = P.rtol;
-    double Hmin = 0.0, Hold = 0.0, Hacc = 0.0, ErrOld = 0.0;
-    double H = P.const_time_step ? (t_end - t_start) : fmin(5e-7, t_end - t_start);
-    double Hnew;
-    double t = t_start;
-    bool Reject = false, FirstStep = true, SkipJac = false, SkipLU = false;
-
-    double sc[R], y0[R], F0[R], Z1[R], Z2[R], Z3[R], DZ1[R], DZ2[R], DZ3[R];
-    double CONT0[R], CONT1[R], CONT2[R];
-    double A[R][N], E1[R][N], E2r[R][N], E2i[R][N];
-    int pos1[R], pos2[R];
-    double rd1[R], rd2r[R], rd2i[R];
-    Perm<N> perm1, perm2;
+  // integrate() prologue, radau2a.c:521-564: a fresh start for every interval.
+  __device__ __forceinline__ void begin(const SolverParams& P, double t_start, double t_stop) {
+    Hold = 0.0; Hacc = 0.0; ErrOld = 0.0;
+    H = P.const_time_step ? (t_stop - t_start) : fmin(5e-7, t_stop - t_start);
+    t = t_start;
+    t_end = t_stop;
+    Reject = false; FirstStep = true; SkipJac = false; SkipLU = false;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      sc[q] = 1.0 / (atol + fabs(y[q]) * rtol);  // scale_init, :88-94
+      sc[q] = 1.0 / (P.atol + fabs(y[q]) * P.rtol);  // scale_init, :88-94
       y0[q] = y[q];
       F0[q] = 0.0; Z1[q] = 0.0; Z2[q] = 0.0; Z3[q] = 0.0;
-      DZ1[q] = 0.0; DZ2[q] = 0.0; DZ3[q] = 0.0;
       CONT0[q] = 0.0; CONT1[q] = 0.0; CONT2[q] = 0.0;
-      rd1[q] = 0.0; rd2r[q] = 0.0; rd2i[q] = 0.0;
-      pos1[q] = -1; pos2[q] = -1;
-#pragma unroll
-      for (int j = 0; j < N; ++j) { A[q][j] = 0.0; E1[q][j] = 0.0; E2r[q][j] = 0.0; E2i[q][j] = 0.0; }
     }
-#pragma unroll
-    for (int i = 0; i < (N + 3) / 4; ++i) { perm1.w[i] = 0; perm2.w[i] = 0; }
-    int info = 0, Nconsecutive = 0, Nsteps = 0;
-    double NewtonRate = pow(2.0, 1.25);
+    Nconsecutive = 0;
+    Nsteps = 0;
+    NewtonRate = 2.3784142300054421;  // pow(2.0, 1.25)
+  }
 
-#pragma unroll 1
-    while (t + kRoundoff < t_end) {
-      if (!Reject) {
-        Mech::template dydt<G, R>(ctx, t, pr, y, F0);
-        cnt[CNT_F]++;
-      }
-      if (!SkipLU) {
-        if (!SkipJac) {
-          Mech::template jacob<G, R>(ctx, t, pr, y, A);
-          cnt[CNT_JAC]++;
-          if constexpr (Mech::kJacInLds) {
+  __device__ __forceinline__ bool finished() const { return !(t + kRoundoff < t_end); }
+
+  // One pass of the step loop, radau2a.c:566-734.  Returns RADAU_CONTINUE,
+  // EC_success once t has reached t_end, or an error code.
+  __device__ __forceinline__ int attempt(Ctx& ctx, const SolverParams& P, double pr, uint32_t (&cnt)[NCNT]) {
+    const int ln = Grp<G>::lane();
+    const double atol = P.atol, rtol = P.rtol;
+    if (!Reject) {
+      Mech::template dydt<G, R>(ctx, t, pr, y, F0);
+      cnt[CNT_F]++;
+    }
+    if (!SkipLU) {
+      if (!SkipJac) {
+        Mech::template jacob<G, R>(ctx, t, pr, y, A);
+        cnt[CNT_JAC]++;
+        if constexpr (Mech::kJacInLds) {
 #pragma unroll
-            for (int q = 0; q < R; ++q)
-              if (RW::valid(ln, q)) {
+          for (int q = 0; q < R; ++q)
+            if (RW::valid(ln, q)) {
 #pragma unroll
-                for (int j = 0; j < N; ++j) ctx.jac[RW::row(ln, q) * N + j] = A[q][j];
-              }
-          }
-        } else if constexpr (Mech::kJacInLds) {
-#pragma unroll
-          for (int q = 0; q < R; ++q) {
-#pragma unroll
-            for (int j = 0; j < N; ++j) A[q][j] = RW::valid(ln, q) ? ctx.jac[RW::row(ln, q) * N + j] : 0.0;
-          }
+              for (int j = 0; j < N; ++j) ctx.jac[RW::row(ln, q) * N + j] = A[q][j];
+            }
         }
-        // RK_Decomp, :256-278
-        const double g1 = rkGamma / H, g2r = rkAlpha / H, g2i = rkBeta / H;
+      } else if constexpr (Mech::kJacInLds && !kColLU) {
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-          const int r = RW::row(ln, q);
 #pragma unroll
-          for (int j = 0; j < N; ++j) {
-            const bool d = (j == r);
-            E1[q][j] = -A[q][j] + (d ? g1 : 0.0);
-            E2r[q][j] = -A[q][j] + (d ? g2r : 0.0);
-            E2i[q][j] = d ? g2i : 0.0;
-          }
-        }
-        lu_real<N, G, R>(E1, pos1, perm1, rd1, info);
-        if (info == 0) lu_cplx<N, G, R>(E2r, E2i, pos2, perm2, rd2r, rd2i, info);
-        cnt[CNT_LU]++;
-        if (info != 0) {
-          cnt[CNT_LU_FAIL]++;
-          Nconsecutive += 1;
-          if (Nconsecutive >= kMaxConsecutiveErrs) return EC_consecutive_steps;
-          H *= 0.5;
-          Reject = true;
-          SkipJac = true;
-          SkipLU = false;
-          continue;
-        } else {
-          Nconsecutive = 0;
+          for (int j = 0; j < N; ++j) A[q][j] = RW::valid(ln, q) ? ctx.jac[RW::row(ln, q) * N + j] : 0.0;
         }
       }
-      Nsteps += 1;
-      cnt[CNT_STEP]++;
-      if (Nsteps >= kMaxNoSteps) return EC_max_steps_exceeded;
-      if (0.1 * fabs(H) <= fabs(t) * kRoundoff) return EC_h_plus_t_equals_h;
-      if (FirstStep) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) { Z1[q] = 0.0; Z2[q] = 0.0; Z3[q] = 0.0; }
-      } else {
-        // RK_Interpolate, :302-314 (third stage keeps the reference's x2 factor)
-        const double rr = H / Hold;
-        const double x1 = 1.0 + rkC0 * rr, x2 = 1.0 + rkC1 * rr, x3 = 1.0 + rkC2 * rr;
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-          Z1[q] = CONT0[q] + x1 * (CONT1[q] + x1 * CONT2[q]);
-          Z2[q] = CONT0[q] + x2 * (CONT1[q] + x2 * CONT2[q]);
-          Z3[q] = CONT0[q] + x2 * (CONT1[q] + x3 * CONT2[q]);
-        }
-      }
-      bool NewtonDone = false;
-      double NewtonIncrementOld = 0.0;
-      double Fac = 0.5;
-      int NewtonIter = 0;
-      double Theta = 0.0;
-      NewtonRate = pow(fmax(NewtonRate, kRoundoff), 0.8);
-
-#pragma unroll 1
-      for (; NewtonIter < kNewtonMaxit; NewtonIter++) {
-        // RK_PrepareRHS, :353-383
-        {
-          double Tm[R], F[R];
-#pragma unroll
-          for (int q = 0; q < R; ++q) { DZ1[q] = Z1[q]; DZ2[q] = Z2[q]; DZ3[q] = Z3[q]; }
-#pragma unroll
-          for (int q = 0; q < R; ++q) Tm[q] = y[q] + Z1[q];
-          Mech::template dydt<G, R>(ctx, t + rkC0 * H, pr, Tm, F);
-          {
-            const double a1 = -H * rkA00, a2 = -H * rkA10, a3 = -H * rkA20;
-#pragma unroll
-            for (int q = 0; q < R; ++q) { DZ1[q] += a1 * F[q]; DZ2[q] += a2 * F[q]; DZ3[q] += a3 * F[q]; }
-          }
-#pragma unroll
-          for (int q = 0; q < R; ++q) Tm[q] = y[q] + Z2[q];
-          Mech::template dydt<G, R>(ctx, t + rkC1 * H, pr, Tm, F);
-          {
-            const double a1 = -H * rkA01, a2 = -H * rkA11, a3 = -H * rkA21;
-#pragma unroll
-            for (int q = 0; q < R; ++q) { DZ1[q] += a1 * F[q]; DZ2[q] += a2 * F[q]; DZ3[q] += a3 * F[q]; }
-          }
-#pragma unroll
-          for (int q = 0; q < R; ++q) Tm[q] = y[q] + Z3[q];
-          Mech::template dydt<G, R>(ctx, t + rkC2 * H, pr, Tm, F);
-          {
-            const double a1 = -H * rkA02, a2 = -H * rkA12, a3 = -H * rkA22;
-#pragma unroll
-            for (int q = 0; q < R; ++q) { DZ1[q] += a1 * F[q]; DZ2[q] += a2 * F[q]; DZ3[q] += a3 * F[q]; }
-          }
-          cnt[CNT_F] += 3;
-        }
-        // RK_Solve, :388-441
-        {
-#pragma unroll
-          for (int q = 0; q < R; ++q) {
-            const double x1 = DZ1[q] / H, x2 = DZ2[q] / H, x3 = DZ3[q] / H;
-            DZ1[q] = rkTiAi00 * x1 + rkTiAi01 * x2 + rkTiAi02 * x3;
-            DZ2[q] = rkTiAi10 * x1 + rkTiAi11 * x2 + rkTiAi12 * x3;
-            DZ3[q] = rkTiAi20 * x1 + rkTiAi21 * x2 + rkTiAi22 * x3;
-          }
-          solve_real<N, G, R>(E1, pos1, perm1, rd1, DZ1);
-          solve_cplx<N, G, R>(E2r, E2i, pos2, perm2, rd2r, rd2i, DZ2, DZ3);
-#pragma unroll
-          for (int q = 0; q < R; ++q) {
-            const double x1 = DZ1[q], x2 = DZ2[q], x3 = DZ3[q];
-            DZ1[q] = rkT00 * x1 + rkT01 * x2 + rkT02 * x3;
-            DZ2[q] = rkT10 * x1 + rkT11 * x2 + rkT12 * x3;
-            DZ3[q] = rkT20 * x1 + rkT21 * x2 + rkT22 * x3;
-          }
-          cnt[CNT_NEWTON]++;
-        }
-        const double d1 = err_norm(sc, DZ1);
-        const double d2 = err_norm(sc, DZ2);
-        const double d3 = err_norm(sc, DZ3);
-        const double NewtonIncrement = sqrt((d1 * d1 + d2 * d2 + d3 * d3) / 3.0);
-        Theta = kThetaMin;
-        if (NewtonIter > 0) {
-          Theta = NewtonIncrement / NewtonIncrementOld;
-          if (Theta < 0.99) {
-            NewtonRate = Theta / (1.0 - Theta);
-          } else {
-            break;
-          }
-          const double NewtonPredictedErr =
-              (NewtonIncrement * pow(Theta, (double)(kNewtonMaxit - NewtonIter - 1))) / (1.0 - Theta);
-          if (NewtonPredictedErr >= kNewtonTol) {
-            const double Qnewton = fmin(10.0, NewtonPredictedErr / kNewtonTol);
-            Fac = 0.8 * pow(Qnewton, -1.0 / ((double)(kNewtonMaxit - NewtonIter)));
-            break;
-          }
-        }
-        NewtonIncrementOld = fmax(NewtonIncrement, kRoundoff);
-#pragma unroll
-        for (int q = 0; q < R; ++q) { Z1[q] -= DZ1[q]; Z2[q] -= DZ2[q]; Z3[q] -= DZ3[q]; }
-        NewtonDone = (NewtonRate * NewtonIncrement <= kNewtonTol);
-        if (NewtonDone) break;
-        if (NewtonIter == kNewtonMaxit - 1) return EC_newton_max_iterations_exceeded;
-      }
-      if (P.const_time_step) {  // :725-733
-        t += H;
-#pragma unroll
-        for (int q = 0; q < R; ++q) y[q] += Z3[q];
-        cnt[CNT_ACCEPT]++;
-        continue;
-      }
-      if (!NewtonDone) {
-        cnt[CNT_NEWTON_FAIL]++;
-        H = Fac * H;
+      const int info = decomp(ctx);
+      cnt[CNT_LU]++;
+      if (info != 0) {
+        cnt[CNT_LU_FAIL]++;
+        Nconsecutive += 1;
+        if (Nconsecutive >= kMaxConsecutiveErrs) return EC_consecutive_steps;
+        H *= 0.5;
         Reject = true;
         SkipJac = true;
         SkipLU = false;
-        continue;
+        return RADAU_CONTINUE;
+      } else {
+        Nconsecutive = 0;
       }
-      // RK_ErrorEstimate, :458-508
-      double Err;
+    }
+    Nsteps += 1;
+    cnt[CNT_STEP]++;
+    if (Nsteps >= kMaxNoSteps) return EC_max_steps_exceeded;
+    if (0.1 * fabs(H) <= fabs(t) * kRoundoff) return EC_h_plus_t_equals_h;
+    if (FirstStep) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) { Z1[q] = 0.0; Z2[q] = 0.0; Z3[q] = 0.0; }
+    } else {
+      // RK_Interpolate, :302-314 (third stage keeps the reference's x2 factor)
+      const double rr = H / Hold;
+      const double x1 = 1.0 + rkC0 * rr, x2 = 1.0 + rkC1 * rr, x3 = 1.0 + rkC2 * rr;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        Z1[q] = CONT0[q] + x1 * (CONT1[q] + x1 * CONT2[q]);
+        Z2[q] = CONT0[q] + x2 * (CONT1[q] + x2 * CONT2[q]);
+        Z3[q] = CONT0[q] + x2 * (CONT1[q] + x3 * CONT2[q]);
+      }
+    }
+    bool NewtonDone = false;
+    double NewtonIncrementOld = 0.0;
+    double Fac = 0.5;
+    int NewtonIter = 0;
+    double Theta = 0.0;
+    NewtonRate = pow(fmax(NewtonRate, kRoundoff), 0.8);
+
+#pragma unroll 1
+    for (; NewtonIter < kNewtonMaxit; NewtonIter++) {
+      double DZ1[R], DZ2[R], DZ3[R];
+      // RK_PrepareRHS, :353-383: R_k = Z_k - H sum_j A_kj f(t + c_j H, y + Z_j),
+      // one callback site for the three stage evaluations
+#pragma unroll
+      for (int q = 0; q < R; ++q) { DZ1[q] = Z1[q]; DZ2[q] = Z2[q]; DZ3[q] = Z3[q]; }
+#pragma unroll 1
+      for (int js = 0; js < 3; ++js) {
+        double Tm[R], F[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) Tm[q] = y[q] + (js == 0 ? Z1[q] : (js == 1 ? Z2[q] : Z3[q]));
+        const double cj = js == 0 ? rkC0 : (js == 1 ? rkC1 : rkC2);
+        Mech::template dydt<G, R>(ctx, t + cj * H, pr, Tm, F);
+        const double a1 = -H * (js == 0 ? rkA00 : (js == 1 ? rkA01 : rkA02));
+        const double a2 = -H * (js == 0 ? rkA10 : (js == 1 ? rkA11 : rkA12));
+        const double a3 = -H * (js == 0 ? rkA20 : (js == 1 ? rkA21 : rkA22));
+#pragma unroll
+        for (int q = 0; q < R; ++q) { DZ1[q] += a1 * F[q]; DZ2[q] += a2 * F[q]; DZ3[q] += a3 * F[q]; }
+      }
+      cnt[CNT_F] += 3;
+      // RK_Solve, :388-441
       {
-        const double HrkE1 = rkE1 / H, HrkE2 = rkE2 / H, HrkE3 = rkE3 / H;
-        double F2[R], Tm[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-          F2[q] = HrkE1 * Z1[q] + HrkE2 * Z2[q] + HrkE3 * Z3[q];
-          Tm[q] = rkE0 * F0[q] + F2[q];
+          const double x1 = DZ1[q] / H, x2 = DZ2[q] / H, x3 = DZ3[q] / H;
+          DZ1[q] = rkTiAi00 * x1 + rkTiAi01 * x2 + rkTiAi02 * x3;
+          DZ2[q] = rkTiAi10 * x1 + rkTiAi11 * x2 + rkTiAi12 * x3;
+          DZ3[q] = rkTiAi20 * x1 + rkTiAi21 * x2 + rkTiAi22 * x3;
         }
-        solve_real<N, G, R>(E1, pos1, perm1, rd1, Tm);
-        cnt[CNT_ES]++;
-        Err = err_norm(sc, Tm);
-        if (Err >= 1.0 && (FirstStep || Reject)) {
-          double F1[R];
+        solve1(DZ1);
+        solve2(DZ2, DZ3);
 #pragma unroll
-          for (int q = 0; q < R; ++q) Tm[q] += y[q];
-          Mech::template dydt<G, R>(ctx, t, pr, Tm, F1);
-          cnt[CNT_F]++;
-#pragma unroll
-          for (int q = 0; q < R; ++q) Tm[q] = F1[q] + F2[q];
-          solve_real<N, G, R>(E1, pos1, perm1, rd1, Tm);
-          cnt[CNT_ES]++;
-          Err = err_norm(sc, Tm);
+        for (int q = 0; q < R; ++q) {
+          const double x1 = DZ1[q], x2 = DZ2[q], x3 = DZ3[q];
+          DZ1[q] = rkT00 * x1 + rkT01 * x2 + rkT02 * x3;
+          DZ2[q] = rkT10 * x1 + rkT11 * x2 + rkT12 * x3;
+          DZ3[q] = rkT20 * x1 + rkT21 * x2 + rkT22 * x3;
+        }
+        cnt[CNT_NEWTON]++;
+      }
+      const double d1 = err_norm(DZ1);
+      const double d2 = err_norm(DZ2);
+      const double d3 = err_norm(DZ3);
+      const double NewtonIncrement = sqrt((d1 * d1 + d2 * d2 + d3 * d3) / 3.0);
+      Theta = kThetaMin;
+      if (NewtonIter > 0) {
+        Theta = NewtonIncrement / NewtonIncrementOld;
+        if (Theta < 0.99) {
+          NewtonRate = Theta / (1.0 - Theta);
+        } else {
+          break;
+        }
+        const double NewtonPredictedErr =
+            (NewtonIncrement * pow(Theta, (double)(kNewtonMaxit - NewtonIter - 1))) / (1.0 - Theta);
+        if (NewtonPredictedErr >= kNewtonTol) {
+          const double Qnewton = fmin(10.0, NewtonPredictedErr / kNewtonTol);
+          Fac = 0.8 * pow(Qnewton, -1.0 / ((double)(kNewtonMaxit - NewtonIter)));
+          break;
         }
       }
-      Fac = pow(Err, (-1.0 / rkELO)) * (1.0 + 2 * kNewtonMaxit) / (NewtonIter + 1.0 + 2 * kNewtonMaxit);
-      Fac = fmin(kFacMax, fmax(kFacMin, Fac));
-      Hnew = Fac * H;
-      if (Err < 1.0) {
-        cnt[CNT_ACCEPT]++;
-        if (!FirstStep) {  // Gustafsson, :676-685
-          double FacGus = kFacSafe * (H / Hacc) * pow(Err * Err / ErrOld, -0.25);
-          FacGus = fmin(kFacMax, fmax(kFacMin, FacGus));
-          Fac = fmin(Fac, FacGus);
-          Hnew = Fac * H;
-        }
-        Hacc = H;
-        ErrOld = fmax(1e-2, Err);
-        FirstStep = false;
-        Hold = H;
-        t += H;
+      NewtonIncrementOld = fmax(NewtonIncrement, kRoundoff);
 #pragma unroll
-        for (int q = 0; q < R; ++q) y[q] += Z3[q];
-        // RK_Make_Interpolate, :283-297
-        {
-          constexpr double den = (rkC2 - rkC1) * (rkC1 - rkC0) * (rkC0 - rkC2);
+      for (int q = 0; q < R; ++q) { Z1[q] -= DZ1[q]; Z2[q] -= DZ2[q]; Z3[q] -= DZ3[q]; }
+      NewtonDone = (NewtonRate * NewtonIncrement <= kNewtonTol);
+      if (NewtonDone) break;
+      if (NewtonIter == kNewtonMaxit - 1) return EC_newton_max_iterations_exceeded;
+    }
+    if (P.const_time_step) {  // :725-733
+      t += H;
 #pragma unroll
-          for (int q = 0; q < R; ++q) {
-            const double z1 = Z1[q], z2 = Z2[q], z3 = Z3[q];
-            CONT0[q] = ((-rkC2 * rkC2 * rkC1 * z1 + z3 * rkC1 * rkC0 * rkC0 + rkC1 * rkC1 * rkC2 * z1 -
-                         rkC1 * rkC1 * rkC0 * z3 + rkC2 * rkC2 * rkC0 * z2 - z2 * rkC2 * rkC0 * rkC0) /
-                        den) - z3;
-            CONT1[q] = -(rkC0 * rkC0 * (z3 - z2) + rkC1 * rkC1 * (z1 - z3) + rkC2 * rkC2 * (z2 - z1)) / den;
-            CONT2[q] = (rkC0 * (z3 - z2) + rkC1 * (z1 - z3) + rkC2 * (z2 - z1)) / den;
-          }
-        }
+      for (int q = 0; q < R; ++q) y[q] += Z3[q];
+      cnt[CNT_ACCEPT]++;
+      return finished() ? EC_success : RADAU_CONTINUE;
+    }
+    if (!NewtonDone) {
+      cnt[CNT_NEWTON_FAIL]++;
+      H = Fac * H;
+      Reject = true;
+      SkipJac = true;
+      SkipLU = false;
+      return RADAU_CONTINUE;
+    }
+    // RK_ErrorEstimate, :458-508: solve E1 x = e0 F0 + F2; on a first or
+    // rejected step with Err >= 1, solve again with f(t, y + x) + F2.
+    double Err;
+    {
+      const double HrkE1 = rkE1 / H, HrkE2 = rkE2 / H, HrkE3 = rkE3 / H;
+      double F2[R], Tm[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) {  // scale(), :74-81
-          sc[q] = 1.0 / (atol + fmax(fabs(y[q]), fabs(y0[q])) * rtol);
-          y0[q] = y[q];
-        }
-        Hnew = fmin(fmax(Hnew, Hmin), t_end - t);
-        if (Reject) Hnew = fmin(Hnew, H);
-        Reject = false;
-        if (t + Hnew / kQmin - t_end >= 0.0) {
-          H = t_end - t;
-        } else {
-          const double Hratio = Hnew / H;
-          SkipLU = (Theta <= kThetaMin) && (Hratio >= kQmin) && (Hratio <= kQmax);
-          if (!SkipLU) H = Hnew;
-        }
-        SkipJac = NewtonIter == 1 || NewtonRate <= kThetaMin;
-      } else {
-        cnt[CNT_REJECT]++;
-        if (FirstStep || Reject) {
-          H = kFacRej * H;
-        } else {
-          H = Hnew;
-        }
-        Reject = true;
-        SkipJac = true;
-        SkipLU = false;
+      for (int q = 0; q < R; ++q) {
+        F2[q] = HrkE1 * Z1[q] + HrkE2 * Z2[q] + HrkE3 * Z3[q];
+        Tm[q] = rkE0 * F0[q] + F2[q];
+      }
+#pragma unroll 1
+      for (int pass = 0;; ++pass) {
+        solve1(Tm);
+        cnt[CNT_ES]++;
+        Err = err_norm(Tm);
+        if (pass == 1 || !(Err >= 1.0 && (FirstStep || Reject))) break;
+        double F1[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) Tm[q] += y[q];
+        Mech::template dydt<G, R>(ctx, t, pr, Tm, F1);
+        cnt[CNT_F]++;
+#pragma unroll
+        for (int q = 0; q < R; ++q) Tm[q] = F1[q] + F2[q];
       }
     }
-    return EC_success;
+    Fac = pow(Err, (-1.0 / rkELO)) * (1.0 + 2 * kNewtonMaxit) / (NewtonIter + 1.0 + 2 * kNewtonMaxit);
+    Fac = fmin(kFacMax, fmax(kFacMin, Fac));
+    double Hnew = Fac * H;
+    if (Err < 1.0) {
+      cnt[CNT_ACCEPT]++;
+      if (!FirstStep) {  // Gustafsson, :676-685
+        double FacGus = kFacSafe * (H / Hacc) * pow(Err * Err / ErrOld, -0.25);
+        FacGus = fmin(kFacMax, fmax(kFacMin, FacGus));
+        Fac = fmin(Fac, FacGus);
+        Hnew = Fac * H;
+      }
+      Hacc = H;
+      ErrOld = fmax(1e-2, Err);
+      FirstStep = false;
+      Hold = H;
+      t += H;
+#pragma unroll
+      for (int q = 0; q < R; ++q) y[q] += Z3[q];
+      // RK_Make_Interpolate, :283-297
+      {
+        constexpr double den = (rkC2 - rkC1) * (rkC1 - rkC0) * (rkC0 - rkC2);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const double z1 = Z1[q], z2 = Z2[q], z3 = Z3[q];
+          CONT0[q] = ((-rkC2 * rkC2 * rkC1 * z1 + z3 * rkC1 * rkC0 * rkC0 + rkC1 * rkC1 * rkC2 * z1 -
+                       rkC1 * rkC1 * rkC0 * z3 + rkC2 * rkC2 * rkC0 * z2 - z2 * rkC2 * rkC0 * rkC0) /
+                      den) - z3;
+          CONT1[q] = -(rkC0 * rkC0 * (z3 - z2) + rkC1 * rkC1 * (z1 - z3) + rkC2 * rkC2 * (z2 - z1)) / den;
+          CONT2[q] = (rkC0 * (z3 - z2) + rkC1 * (z1 - z3) + rkC2 * (z2 - z1)) / den;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) {  // scale(), :74-81
+        sc[q] = 1.0 / (atol + fmax(fabs(y[q]), fabs(y0[q])) * rtol);
+        y0[q] = y[q];
+      }
+      Hnew = fmin(fmax(Hnew, 0.0), t_end - t);  // Hmin = 0
+      if (Reject) Hnew = fmin(Hnew, H);
+      Reject = false;
+      if (t + Hnew / kQmin - t_end >= 0.0) {
+        H = t_end - t;
+      } else {
+        const double Hratio = Hnew / H;
+        SkipLU = (Theta <= kThetaMin) && (Hratio >= kQmin) && (Hratio <= kQmax);
+        if (!SkipLU) H = Hnew;
+      }
+      SkipJac = NewtonIter == 1 || NewtonRate <= kThetaMin;
+      return finished() ? EC_success : RADAU_CONTINUE;
+    } else {
+      cnt[CNT_REJECT]++;
+      if (FirstStep || Reject) {
+        H = kFacRej * H;
+      } else {
+        H = Hnew;
+      }
+      Reject = true;
+      SkipJac = true;
+      SkipLU = false;
+      return RADAU_CONTINUE;
+    }
   }
 };
 

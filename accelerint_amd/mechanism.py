@@ -112,7 +112,7 @@ def _try_generate(rng, F, ns, nr, seed):
     a0 = 2.5 + 0.9 * (atoms - 1) + rng.uniform(0.0, 1.0, ns)
     a1 = rng.uniform(0.0, 1.5e-3, ns) * atoms
     a2 = -rng.uniform(0.0, 2.0e-7, ns) * atoms
-    bh = rng.uniform(-1.2e4, 1.2e4, ns)
+    bh = rng.uniform(-6.0e3, 6.0e3, ns)
     bs = rng.uniform(5.0, 25.0, ns)
     thermo = np.stack([a0, a1, a2, bh, bs], 1)
 
