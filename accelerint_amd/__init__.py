@@ -25,6 +25,8 @@ _lib = None
 
 MECH_VAN_DER_POL = 0
 MECH_KINETICS = 2
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
 NCNT = 10
 CNT_NAMES = ("step", "accept", "reject", "newton_fail", "lu", "lu_fail", "newton", "f", "jac", "es")
 
@@ -36,7 +38,7 @@ EXPORTS = (
     "accelerInt_set_tolerances", "accelerInt_set_const_time_step", "accelerInt_set_abort_on_error",
     "accelerInt_get_status", "accelerInt_get_counters", "accelerInt_reset_counters",
     "accelerInt_num_states", "accelerInt_integrate_device", "accelerInt_synchronize",
-    "accelerInt_last_error", "accelerInt_eval_callbacks",
+    "accelerInt_last_error", "accelerInt_eval_callbacks", "accelerInt_set_precision",
 )
 
 
@@ -80,6 +82,7 @@ def lib():
         "accelerInt_synchronize": ([], i),
         "accelerInt_last_error": ([], ctypes.c_char_p),
         "accelerInt_eval_callbacks": ([i, d, v, v, v, v], i),
+        "accelerInt_set_precision": ([i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -127,6 +130,12 @@ def set_tolerances(atol: float, rtol: float):
 
 def set_const_time_step(on: bool):
     lib().accelerInt_set_const_time_step(int(bool(on)))
+
+
+def set_precision(mode: int):
+    """PRECISION_FP64 (default) or PRECISION_MIXED (FP32 iteration-matrix LU and
+    Newton solves; FP64 residuals, error estimate, step control and state)."""
+    _check(lib().accelerInt_set_precision(int(mode)), "accelerInt_set_precision")
 
 
 def set_abort_on_error(on: bool):

@@ -32,6 +32,9 @@
 #ifndef ACC_K14_WPE
 #define ACC_K14_WPE 2
 #endif
+#ifndef ACC_K54_WPE
+#define ACC_K54_WPE 1
+#endif
 
 namespace acc {
 
@@ -53,7 +56,7 @@ __device__ __forceinline__ int grab_condition(int* work) {
   return Grp<G>::bcast_i(c, 0);
 }
 
-template <class Mech, int G, int R, int BLOCK, int WPE>
+template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
     int num, double t_start, double t_end, double step, const double* __restrict__ pr,
     double* __restrict__ y, int* __restrict__ codes, uint32_t* __restrict__ counters,
@@ -62,9 +65,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)
   constexpr int N = Mech::N;
   const int g = threadIdx.x / G;
   const int ln = threadIdx.x % G;
+  using Integrator = Radau<Mech, G, R, MT>;
   Mech::block_init(mp, lds_dyn);
-  typename Mech::Ctx ctx(mp, g, lds_dyn);
-  Radau<Mech, G, R> st;
+  typename Mech::Ctx ctx(mp, g, lds_dyn, Integrator::kNeedsScratch);
+  Integrator st;
   uint32_t cnt[NCNT];
   double prc = 0.0, t_next = t_end;
   int k = 0;
@@ -141,9 +145,19 @@ __global__ __launch_bounds__(BLOCK) void k_eval_callbacks(int num, double t,
     const int r = ln + G * q;
     yl[q] = (r < N) ? y[(size_t)c * N + r] : 0.0;
   }
-  typename Mech::Ctx ctx(mp, g, lds_dyn);
+  typename Mech::Ctx ctx(mp, g, lds_dyn, false);
   Mech::template dydt<G, R>(ctx, t, pr[c], yl, f);
-  Mech::template jacob<G, R>(ctx, t, pr[c], yl, J);
+  if constexpr (Mech::kJacInLds) {
+    Mech::template jacob<G, R>(ctx, t, pr[c], yl);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = ln + G * q;
+#pragma unroll
+      for (int j = 0; j < N; ++j) J[q][j] = (r < N) ? ctx.jac[r * N + j] : 0.0;
+    }
+  } else {
+    Mech::template jacob<G, R>(ctx, t, pr[c], yl, J);
+  }
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int r = ln + G * q;
@@ -155,14 +169,25 @@ __global__ __launch_bounds__(BLOCK) void k_eval_callbacks(int num, double t,
   }
 }
 
+// Kernels whose dynamic LDS exceeds 64 KiB declare it before launch.
+template <class K>
+hipError_t allow_lds(K kern, size_t lds) {
+  if (lds <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds);
+}
+
 template <class Mech, int G, int R, int BLOCK>
 hipError_t launch_eval(int num, double t, const double* pr, const double* y, double* dy,
                        double* jac, const typename Mech::Params& mp, hipStream_t s) {
   constexpr int GPB = BLOCK / G;
   const int blocks = (num + GPB - 1) / GPB;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_eval_callbacks<Mech, G, R, BLOCK>), dim3(blocks), dim3(BLOCK),
-                     Mech::lds_bytes(mp, GPB), s, num, t, pr, y, dy, jac, mp);
+  auto kern = k_eval_callbacks<Mech, G, R, BLOCK>;
+  const size_t lds = Mech::lds_bytes(mp, GPB, false);
+  hipError_t e = allow_lds(kern, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, num, t, pr, y, dy, jac, mp);
   return hipGetLastError();
 }
 
@@ -182,31 +207,51 @@ __global__ void k_cond_to_cm(const double* __restrict__ src, double* __restrict_
   dst[(size_t)i * ld + c] = src[(size_t)c * n + i];
 }
 
+// Why the last launch was refused before reaching the device (empty if none).
+std::string g_launch_msg;
+
 // Persistent grid: as many blocks as fit on the device at once (capped by the
-// number of conditions).
-template <class Mech, int G, int R, int BLOCK, int WPE>
+// number of conditions).  The group's LDS share must fit the device's
+// per-block limit and leave at least one resident block per CU.
+template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
 hipError_t launch_radau(int num, double t_start, double t_end, double step, const double* pr,
                         double* y, int* codes, uint32_t* cnt, int* work, const SolverParams& P,
                         const typename Mech::Params& mp, hipStream_t s) {
   constexpr int GPB = BLOCK / G;
+  auto kern = k_radau<Mech, G, R, BLOCK, WPE, MT>;
   if (num <= 0) return hipSuccess;
-  const size_t lds = Mech::lds_bytes(mp, GPB);
-  static int resident = 0;  // blocks per device that fit at once, per instantiation
-  if (resident == 0) {
-    int dev = 0, cus = 0, per_cu = 0;
+  const size_t lds = Mech::lds_bytes(mp, GPB, Radau<Mech, G, R, MT>::kNeedsScratch);
+  static int resident = 0;     // blocks per device that fit at once, per instantiation
+  static size_t lds_ok = 0;    // LDS size the residency was computed for
+  if (resident == 0 || lds_ok != lds) {
+    int dev = 0, cus = 0, per_cu = 0, max_lds = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_radau<Mech, G, R, BLOCK, WPE>,
-                                                       BLOCK, lds);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
     if (e != hipSuccess) return e;
-    resident = (cus > 0 ? cus : 1) * (per_cu > 0 ? per_cu : 1);
+    if (lds > (size_t)max_lds) {
+      char buf[256];
+      snprintf(buf, sizeof buf, "k_radau needs %zu B of LDS per block (%d lane groups), device allows %d",
+               lds, GPB, max_lds);
+      g_launch_msg = buf;
+      return hipErrorInvalidConfiguration;
+    }
+    e = allow_lds(kern, lds);
+    if (e != hipSuccess) return e;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, BLOCK, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu <= 0) {
+      g_launch_msg = "k_radau: no block fits on a compute unit";
+      return hipErrorInvalidConfiguration;
+    }
+    resident = (cus > 0 ? cus : 1) * per_cu;
+    lds_ok = lds;
   }
   int blocks = (num + GPB - 1) / GPB;
   if (blocks > resident) blocks = resident;
   hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work);
-  hipLaunchKernelGGL((k_radau<Mech, G, R, BLOCK, WPE>), dim3(blocks), dim3(BLOCK), lds, s, num,
-                     t_start, t_end, step, pr, y, codes, cnt, work, P, mp);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, num, t_start, t_end, step, pr, y, codes,
+                     cnt, work, P, mp);
   return hipGetLastError();
 }
 
@@ -220,6 +265,7 @@ namespace {
 struct State {
   int mech = ACCELERINT_MECH_VAN_DER_POL;
   int n = 2;
+  int precision = ACCELERINT_PRECISION_FP64;
   SolverParams P{1e-10, 1e-6, 0};
   int abort_on_error = 1;
   int device = -1;
@@ -295,20 +341,35 @@ bool alloc_buffers(int cap) {
   return true;
 }
 
-hipError_t launch(int num, double t_start, double t_end, double step, const double* pr, double* y,
-                  int* codes, uint32_t* cnt, hipStream_t s) {
+// Lane-group shape per system size: van der Pol one condition per lane (two
+// rows each); 14 states a 16-lane group (four conditions per wave); 54 states
+// a whole wave per condition, four waves per block sharing the LDS tables.
+template <class MT>
+hipError_t launch_mt(int num, double t_start, double t_end, double step, const double* pr, double* y,
+                     int* codes, uint32_t* cnt, hipStream_t s) {
   switch (S.mech) {
     case ACCELERINT_MECH_VAN_DER_POL:
-      return launch_radau<MechVdP, 1, 2, 256, ACC_VDP_WPE>(num, t_start, t_end, step, pr, y, codes, cnt, S.d_work,
-                                                 S.P, MechVdP::Params{}, s);
+      return launch_radau<MechVdP, 1, 2, 256, ACC_VDP_WPE, MT>(num, t_start, t_end, step, pr, y, codes, cnt,
+                                                               S.d_work, S.P, MechVdP::Params{}, s);
     case ACCELERINT_MECH_KINETICS:
       if (S.n == 14)
-        return launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE>(num, t_start, t_end, step, pr, y, codes,
-                                                           cnt, S.d_work, S.P, S.kp14, s);
+        return launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE, MT>(num, t_start, t_end, step, pr, y,
+                                                                         codes, cnt, S.d_work, S.P, S.kp14, s);
+      if (S.n == 54)
+        return launch_radau<MechKinetics54, 64, 1, 256, ACC_K54_WPE, MT>(num, t_start, t_end, step, pr, y,
+                                                                         codes, cnt, S.d_work, S.P, S.kp54, s);
       return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch(int num, double t_start, double t_end, double step, const double* pr, double* y,
+                  int* codes, uint32_t* cnt, hipStream_t s) {
+  g_launch_msg.clear();
+  if (S.precision == ACCELERINT_PRECISION_MIXED)
+    return launch_mt<float>(num, t_start, t_end, step, pr, y, codes, cnt, s);
+  return launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, s);
 }
 
 hipError_t launch_eval_any(int num, double t, const double* pr, const double* y, double* dy,
@@ -318,6 +379,7 @@ hipError_t launch_eval_any(int num, double t, const double* pr, const double* y,
       return launch_eval<MechVdP, 1, 2, 256>(num, t, pr, y, dy, jac, MechVdP::Params{}, s);
     case ACCELERINT_MECH_KINETICS:
       if (S.n == 14) return launch_eval<MechKinetics14, 16, 1, 128>(num, t, pr, y, dy, jac, S.kp14, s);
+      if (S.n == 54) return launch_eval<MechKinetics54, 64, 1, 256>(num, t, pr, y, dy, jac, S.kp54, s);
       return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
@@ -331,6 +393,14 @@ int run_steps(int num, double t_start, double t_end, double stepsize, double* y,
   const double step = stepsize < 0 ? t_end - t_start : stepsize;
   if (!(t_start + kRoundoff < t_end) || num <= 0) return 0;
   hipError_t e = launch(num, t_start, t_end, step, pr, y, codes, cnt, s);
+  if (e != hipSuccess && !g_launch_msg.empty()) {
+    set_err("%s", g_launch_msg.c_str());
+    if (S.abort_on_error) {
+      fprintf(stderr, "%s\n", g_launch_msg.c_str());
+      exit((int)e);
+    }
+    return -1;
+  }
   return hip_ok(e, "k_radau launch") ? 0 : -1;
 }
 
@@ -398,6 +468,15 @@ int accelerInt_set_tolerances(double atol, double rtol) {
   }
   S.P.atol = atol;
   S.P.rtol = rtol;
+  return 0;
+}
+
+int accelerInt_set_precision(int mode) {
+  if (mode != ACCELERINT_PRECISION_FP64 && mode != ACCELERINT_PRECISION_MIXED) {
+    set_err("unknown precision mode %d", mode);
+    return -1;
+  }
+  S.precision = mode;
   return 0;
 }
 

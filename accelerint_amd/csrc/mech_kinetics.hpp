@@ -45,15 +45,20 @@ constexpr double kKinTmax = 1.0e4;
 constexpr double kKinSmin = 1.0e-3;
 constexpr double kKinCpmin = 1.0;
 
-// Device view of a network: one packed blob (built by KineticsHost) that every
-// block copies into the front of its LDS at kernel start.  Byte offsets:
+// Device view of a network: one packed blob (built by KineticsHost).  Its
+// first lds_bytes (the tables read at data-dependent indices) are copied by
+// every block into the front of its LDS at kernel start; the per-reaction rate
+// parameters, which each lane reads only for its own reactions, stay in global
+// memory (L2 resident).  Byte offsets:
 //   [off_invW] double invW[ns]      [off_W] double W[ns]
-//   [off_th]   double thermo[5 ns]  [off_rc] double rc[3 nr] (lnA, beta, EaR)
-//   [off_sl]   uint64 slots[nr]     byte a = species of slot a, 0xff = empty
+//   [off_th]   double thermo[5 ns]  [off_sl] uint64 slots[nr] (byte a = species of slot a, 0xff = empty)
 //   [off_co]   int csr_off[ns + 1]  [off_ce] int csr[nnz] = (reaction << 8) | (nu & 0xff)
+//   ---- end of the LDS part (lds_bytes) ----
+//   [off_rc]   double rc[3 nr] (lnA, beta, EaR)
 struct KinParams {
   const double* blob;  // device copy
   int blob_bytes;      // multiple of 16
+  int lds_bytes;       // multiple of 16, the LDS-resident prefix
   int ns, nr, nnz;
   int off_invW, off_W, off_th, off_rc, off_sl, off_co, off_ce;
 };
@@ -65,20 +70,23 @@ struct MechKinetics {
   using Params = KinParams;
   static constexpr bool kJacInLds = true;
 
-  // per group: C, gRT, hR, cpR, col [NS each]; kf, kr, q, qT [nr each];
-  // scratch [N*N] (dwdC rows while the Jacobian is assembled, the LU
-  // transposition while the iteration matrices are factored); parked
-  // Jacobian [N*N]
-  __host__ __device__ static size_t group_doubles(int nr) {
-    return (size_t)5 * NS + (size_t)4 * nr + (size_t)2 * N * N;
+  // Callbacks: dydt returns the rows a lane owns; jacob writes the dense
+  // Jacobian into the group's LDS (kJacInLds), where the integrator keeps it.
+  //
+  // per group: C, gRT, hR, cpR, col [NS each]; kf, kr, q, qT [nr each]; the
+  // Jacobian buffer [N*N] (d wdot / d C rows while the Jacobian is assembled,
+  // then the parked Jacobian); optionally a scratch [N*N] for integrators that
+  // transpose their factors through LDS.
+  __host__ __device__ static size_t group_doubles(int nr, bool scratch) {
+    return (size_t)5 * NS + (size_t)4 * nr + (size_t)(scratch ? 2 : 1) * N * N;
   }
-  static size_t lds_bytes(const Params& p, int groups) {
-    return (size_t)p.blob_bytes + groups * group_doubles(p.nr) * sizeof(double);
+  static size_t lds_bytes(const Params& p, int groups, bool scratch) {
+    return (size_t)p.lds_bytes + groups * group_doubles(p.nr, scratch) * sizeof(double);
   }
 
   // Whole block: copy the network tables into LDS (before any divergence).
   __device__ __forceinline__ static void block_init(const Params& p, double* lds) {
-    const int n16 = p.blob_bytes / 16;
+    const int n16 = p.lds_bytes / 16;
     const int4* src = reinterpret_cast<const int4*>(p.blob);
     int4* dst = reinterpret_cast<int4*>(lds);
     for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
@@ -106,16 +114,16 @@ struct MechKinetics {
     double* Jc;
     double* scratch;
     double* jac;
-    __device__ Ctx(const Params& pp, int g, double* lds) : nr(pp.nr) {
+    __device__ Ctx(const Params& pp, int g, double* lds, bool with_scratch) : nr(pp.nr) {
       const char* t = reinterpret_cast<const char*>(lds);
       invW = reinterpret_cast<const double*>(t + pp.off_invW);
       W = reinterpret_cast<const double*>(t + pp.off_W);
       thermo = reinterpret_cast<const double*>(t + pp.off_th);
-      rc = reinterpret_cast<const double*>(t + pp.off_rc);
+      rc = reinterpret_cast<const double*>(reinterpret_cast<const char*>(pp.blob) + pp.off_rc);
       slots = reinterpret_cast<const unsigned long long*>(t + pp.off_sl);
       csr_off = reinterpret_cast<const int*>(t + pp.off_co);
       csr = reinterpret_cast<const int*>(t + pp.off_ce);
-      double* b = lds + pp.blob_bytes / 8 + (size_t)g * group_doubles(pp.nr);
+      double* b = lds + pp.lds_bytes / 8 + (size_t)g * group_doubles(pp.nr, with_scratch);
       C = b; b += NS;
       gRT = b; b += NS;
       hR = b; b += NS;
@@ -125,10 +133,12 @@ struct MechKinetics {
       kr = b; b += pp.nr;
       q = b; b += pp.nr;
       qT = b; b += pp.nr;
+      // d wdot / d C is fully consumed (column sums) before the caller parks
+      // the Jacobian rows, so the two share one buffer
       Jc = b;
-      scratch = b;
-      b += N * N;
       jac = b;
+      b += N * N;
+      scratch = with_scratch ? b : nullptr;
     }
   };
 
@@ -190,10 +200,22 @@ struct MechKinetics {
     const double lnC0 = kKinLnC0R - m.lnT;  // ln(1e-6 P_atm / (R T))
     const double lnTr = m.lnT - kKinLnTref;
     const double iT = m.iT;
+    // rate parameters come from global memory, one reaction ahead
+    double nlnA = 0.0, nbeta = 0.0, nEaR = 0.0;
+    if (ln < c.nr) {
+      nlnA = c.rc[3 * ln];
+      nbeta = c.rc[3 * ln + 1];
+      nEaR = c.rc[3 * ln + 2];
+    }
 #pragma unroll 1
     for (int j = ln; j < c.nr; j += G) {
       const unsigned long long sw = c.slots[j];
-      const double lnA = c.rc[3 * j], beta = c.rc[3 * j + 1], EaR = c.rc[3 * j + 2];
+      const double lnA = nlnA, beta = nbeta, EaR = nEaR;
+      if (j + G < c.nr) {
+        nlnA = c.rc[3 * (j + G)];
+        nbeta = c.rc[3 * (j + G) + 1];
+        nEaR = c.rc[3 * (j + G) + 2];
+      }
       double sum_nu_g = 0.0, sum_nu_h = 0.0;
       int dnu = 0;
       double pf = 1.0, pr = 1.0;
@@ -265,9 +287,12 @@ struct MechKinetics {
     group_lds_sync();  // LDS is reused by the next evaluation
   }
 
+  // Dense Jacobian, written as rows into the group's parked-Jacobian buffer:
+  // c.jac[r*N + j] = d f_r / d y_j (row 0 = T).  The buffer doubles as the
+  // d wdot / d C scratch, so each lane keeps its own d wdot / d C row in
+  // registers until every lane has finished reading the scratch.
   template <int G, int R>
-  __device__ __forceinline__ static void jacob(Ctx& c, double, double P, const double (&y)[R],
-                                               double (&J)[R][N]) {
+  __device__ __forceinline__ static void jacob(Ctx& c, double, double P, const double (&y)[R]) {
     const int ln = Grp<G>::lane();
     Mix m;
     prologue<G, R>(c, P, y, m);
@@ -275,9 +300,14 @@ struct MechKinetics {
     const double S = m.S, Ctot = m.Ctot, iT = m.iT;
     const double D = S / Ctot;
     double Qp = 0.0, dQp = 0.0;
+    double jr[R][NS];
+    double rsv[R], wv[R], wTv[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int r = ln + G * q;
+      rsv[q] = wv[q] = wTv[q] = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; ++i) jr[q][i] = 0.0;
       if (r >= 1 && r < N) {
         const int s = r - 1;
         double* row = c.Jc + (size_t)s * NS;
@@ -313,22 +343,19 @@ struct MechKinetics {
             }
           }
         }
-        // own row back into registers; r_s = sum_m Jc[s][m] C_m
-        double jr[NS];
+        // own row into registers; r_s = sum_m Jc[s][m] C_m
         double rs = 0.0;
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
-          jr[i] = row[i];
-          rs += jr[i] * c.C[i];
+          jr[q][i] = row[i];
+          rs += jr[q][i] * c.C[i];
         }
-        const double Ws = c.W[s];
-        const double corr = (rs - w) / Ctot;
-#pragma unroll
-        for (int i = 0; i < NS; ++i) J[q][1 + i] = Ws * c.invW[i] * (jr[i] - corr);
-        J[q][0] = Ws * D * (wT + (w - rs) * iT);
+        rsv[q] = rs;
+        wv[q] = w;
+        wTv[q] = wT;
         const double hs = c.hR[s];
 #pragma unroll
-        for (int i = 0; i < NS; ++i) row[i] = hs * (Ctot * jr[i] - rs);
+        for (int i = 0; i < NS; ++i) row[i] = hs * (Ctot * jr[q][i] - rs);
         Qp += hs * w;
         dQp += c.cpR[s] * w + hs * (wT - rs * iT);
       }
@@ -348,22 +375,28 @@ struct MechKinetics {
         c.col[i] = sum;
       }
     }
-    group_lds_sync();
+    group_lds_sync();  // the scratch is dead from here on: rows go to c.jac
     const double cpm = m.cpm;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (ln + G * q == 0) {
+      const int r = ln + G * q;
+      if (r == 0) {
         const double QD = Q * D;
 #pragma unroll
         for (int i = 0; i < NS; ++i) {
           const double iw = c.invW[i];
           const double dQdYi = kKinR * c.col[i] * iw / S;
-          J[q][1 + i] = -(D * dQdYi + Q * iw / Ctot) / cpm + QD * kKinR * c.cpR[i] * iw / (cpm * cpm);
+          c.jac[1 + i] = -(D * dQdYi + Q * iw / Ctot) / cpm + QD * kKinR * c.cpR[i] * iw / (cpm * cpm);
         }
-        J[q][0] = -(D * dQdT + QD * iT) / cpm + QD * m.dcpm / (cpm * cpm);
-      } else if (ln + G * q >= N) {
+        c.jac[0] = -(D * dQdT + QD * iT) / cpm + QD * m.dcpm / (cpm * cpm);
+      } else if (r < N) {
+        const int s = r - 1;
+        const double Ws = c.W[s];
+        const double corr = (rsv[q] - wv[q]) / Ctot;
+        double* out = c.jac + (size_t)r * N;
 #pragma unroll
-        for (int i = 0; i < N; ++i) J[q][i] = 0.0;
+        for (int i = 0; i < NS; ++i) out[1 + i] = Ws * c.invW[i] * (jr[q][i] - corr);
+        out[0] = Ws * D * (wTv[q] + (wv[q] - rsv[q]) * iT);
       }
     }
     group_lds_sync();
@@ -473,10 +506,11 @@ struct KineticsHost {
     dev.off_invW = section(invW.data(), ns * sizeof(double));
     dev.off_W = section(W.data(), ns * sizeof(double));
     dev.off_th = section(thermo.data(), 5 * (size_t)ns * sizeof(double));
-    dev.off_rc = section(rc.data(), 3 * (size_t)nr * sizeof(double));
     dev.off_sl = section(sl.data(), (size_t)nr * sizeof(unsigned long long));
     dev.off_co = section(csr_off.data(), (size_t)(ns + 1) * sizeof(int));
     dev.off_ce = section(ce.data(), (size_t)ce.size() * sizeof(int));
+    dev.lds_bytes = (int)blob.size();
+    dev.off_rc = section(rc.data(), 3 * (size_t)nr * sizeof(double));
     dev.blob_bytes = (int)blob.size();
     if (hipMalloc(&dmem, blob.size()) != hipSuccess) {
       *msg = "hipMalloc failed for kinetics tables";

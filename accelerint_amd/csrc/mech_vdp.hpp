@@ -13,12 +13,12 @@ struct MechVdP {
   static constexpr int N = 2;
   static constexpr bool kJacInLds = false;
   struct Params {};
-  static size_t lds_bytes(const Params&, int) { return 0; }
+  static size_t lds_bytes(const Params&, int, bool) { return 0; }
   __device__ __forceinline__ static void block_init(const Params&, double*) {}
   struct Ctx {
     double* jac = nullptr;
     double* scratch = nullptr;
-    __device__ Ctx(const Params&, int /*group*/, double* /*lds*/) {}
+    __device__ Ctx(const Params&, int /*group*/, double* /*lds*/, bool /*scratch*/) {}
   };
 
   template <int G, int R>

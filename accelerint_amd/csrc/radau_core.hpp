@@ -122,6 +122,16 @@ struct Grp {
       return __shfl(v, src, G);
     }
   }
+  __device__ __forceinline__ static float bcast(float v, int src) {
+    if constexpr (G == 1) {
+      return v;
+    } else if constexpr (G == 64) {
+      int s = __builtin_amdgcn_readfirstlane(src);
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), s));
+    } else {
+      return __shfl(v, src, G);
+    }
+  }
   __device__ __forceinline__ static int bcast_i(int v, int src) {
     if constexpr (G == 1) {
       return v;
@@ -141,15 +151,12 @@ struct Grp {
     }
   }
   __device__ __forceinline__ static double xor_(double v, int m) { return __shfl_xor(v, m, G); }
+  __device__ __forceinline__ static float xor_(float v, int m) { return __shfl_xor(v, m, G); }
   __device__ __forceinline__ static int xor_i(int v, int m) { return __shfl_xor(v, m, G); }
 
   // Butterfly all-reduce: partners add the same two operands, so every lane
   // of the group ends with a bit-identical result.
-  __device__ __forceinline__ static double sum(double v) {
-#pragma unroll
-    for (int m = 1; m < G; m <<= 1) v += xor_(v, m);
-    return v;
-  }
+  __device__ __forceinline__ static double sum(double v);
 };
 
 // Row ownership helpers.
@@ -161,9 +168,9 @@ struct Rows {
 };
 
 // Select x[q] for a runtime q without dynamic register indexing.
-template <int R>
-__device__ __forceinline__ double sel(const double (&x)[R], int q) {
-  double v = x[0];
+template <int R, class T>
+__device__ __forceinline__ T sel(const T (&x)[R], int q) {
+  T v = x[0];
 #pragma unroll
   for (int i = 1; i < R; ++i) v = (q == i) ? x[i] : v;
   return v;
@@ -180,31 +187,102 @@ struct Perm {
   __device__ __forceinline__ int get(int k) const { return (int)((w[k >> 2] >> (8 * (k & 3))) & 0xffu); }
 };
 
+// Smallest normal number of the matrix scalar type (LAPACK's sfmin test).
+template <class T>
+__device__ __forceinline__ constexpr T min_normal() {
+  if constexpr (std::is_same<T, float>::value) return 1.17549435082228750797e-38f;
+  else return kDblMin;
+}
+
 // Smith's complex reciprocal 1/(c + i d).
-__device__ __forceinline__ void crecip(double c, double d, double& re, double& im) {
+template <class T>
+__device__ __forceinline__ void crecip(T c, T d, T& re, T& im) {
   if (fabs(c) >= fabs(d)) {
-    double r = d / c, den = c + d * r;
-    re = 1.0 / den;
+    T r = d / c, den = c + d * r;
+    re = T(1) / den;
     im = -r / den;
   } else {
-    double r = c / d, den = c * r + d;
+    T r = c / d, den = c * r + d;
     re = r / den;
-    im = -1.0 / den;
+    im = T(-1) / den;
   }
 }
 
 // Argmax over the group of (value, key) with key = (LAPACK position << 8) | row;
 // larger value wins, equal values go to the smaller position (first maximum).
-template <int G>
-__device__ __forceinline__ void group_argmax(double& bv, int& bkey) {
+// DPP lane exchanges inside a 16-lane row: quad_perm [1,0,3,2] (xor 1),
+// quad_perm [2,3,0,1] (xor 2), row_half_mirror (i <-> 7 - i in each 8) and
+// row_mirror (i <-> 15 - i).  Applied in that order, a commutative pairwise
+// combine leaves every lane of the row with the combination of all 16.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_t(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_t(double v) {
+  return __hiloint2double(dpp_i<CTRL>(__double2hiint(v)), dpp_i<CTRL>(__double2loint(v)));
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+// (value, key) order of the pivot search: larger value, then smaller key.
+// Bitwise, not short-circuit, so the exchange stays branch-free.
+template <class T>
+__device__ __forceinline__ void argmax_merge(T& bv, int& bkey, T ov, int ok) {
+  const bool better = (ov > bv) | ((ov == bv) & (ok < bkey));
+  bv = better ? ov : bv;
+  bkey = better ? ok : bkey;
+}
+
+// Argmax over the group of (value, key) with key = (LAPACK position << 8) | row;
+// larger value wins, equal values go to the smaller position (first maximum).
+// The order is total on non-NaN values, so every pairing network returns the
+// same winner: 16- and 64-lane groups use DPP within rows (plus, for 64, the
+// four row winners read through SGPRs); other widths an xor butterfly.
+template <int G, class T>
+__device__ __forceinline__ void group_argmax(T& bv, int& bkey) {
+  if constexpr (G == 16 || G == 64) {
+    argmax_merge(bv, bkey, dpp_t<kDppXor1>(bv), dpp_i<kDppXor1>(bkey));
+    argmax_merge(bv, bkey, dpp_t<kDppXor2>(bv), dpp_i<kDppXor2>(bkey));
+    argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
+    argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
+    if constexpr (G == 64) {
+      T v0 = Grp<64>::bcast(bv, 0), v1 = Grp<64>::bcast(bv, 16);
+      T v2 = Grp<64>::bcast(bv, 32), v3 = Grp<64>::bcast(bv, 48);
+      int k0 = Grp<64>::bcast_i(bkey, 0), k1 = Grp<64>::bcast_i(bkey, 16);
+      int k2 = Grp<64>::bcast_i(bkey, 32), k3 = Grp<64>::bcast_i(bkey, 48);
+      argmax_merge(v0, k0, v1, k1);
+      argmax_merge(v2, k2, v3, k3);
+      argmax_merge(v0, k0, v2, k2);
+      bv = v0;
+      bkey = k0;
+    }
+  } else {
 #pragma unroll
-  for (int m = 1; m < G; m <<= 1) {
-    double ov = Grp<G>::xor_(bv, m);
-    int ok = Grp<G>::xor_i(bkey, m);
-    bool better = (ov > bv) || (ov == bv && ok < bkey);
-    bv = better ? ov : bv;
-    bkey = better ? ok : bkey;
+    for (int m = 1; m < G; m <<= 1) argmax_merge(bv, bkey, Grp<G>::xor_(bv, m), Grp<G>::xor_i(bkey, m));
   }
+}
+
+// Group all-reduce of a sum.  16- and 64-lane groups: DPP pairs within rows
+// (each pair adds the same two operands, so the lanes of a row agree bit for
+// bit), then for 64 the four row sums in a fixed order from SGPRs; other
+// widths: xor butterfly.
+template <int G>
+__device__ __forceinline__ double Grp<G>::sum(double v) {
+  if constexpr (G == 16 || G == 64) {
+    v += dpp_t<kDppXor1>(v);
+    v += dpp_t<kDppXor2>(v);
+    v += dpp_t<kDppHalfMirror>(v);
+    v += dpp_t<kDppMirror>(v);
+    if constexpr (G == 64) v = (bcast(v, 0) + bcast(v, 16)) + (bcast(v, 32) + bcast(v, 48));
+  } else {
+#pragma unroll
+    for (int m = 1; m < G; m <<= 1) v += xor_(v, m);
+  }
+  return v;
 }
 
 // ---- real LU (dgetf2 semantics, implicit interchanges) ---------------------
@@ -217,24 +295,24 @@ __device__ __forceinline__ void group_argmax(double& bv, int& bkey) {
 // eliminated, padding rows) get a zero multiplier, so the update is
 // branch-free and each pivot-row entry is broadcast just before the column
 // update that consumes it.
-template <int N, int G, int R>
-__device__ __forceinline__ void lu_real(double (&a)[R][N], int (&pos)[R], Perm<N>& perm,
-                                        double (&rd)[R], int& info) {
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void lu_real(T (&a)[R][N], int (&pos)[R], Perm<N>& perm,
+                                        T (&rd)[R], int& info) {
   const int ln = Grp<G>::lane();
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     pos[q] = Rows<N, G, R>::valid(ln, q) ? Rows<N, G, R>::row(ln, q) : -1;
-    rd[q] = 0.0;
+    rd[q] = T(0);
   }
   info = 0;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    double bv = -1.0;
+    T bv = T(-1);
     int bkey = 0x7fffffff;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const bool act = pos[q] >= k;
-      const double v = fabs(a[q][k]);
+      const T v = fabs(a[q][k]);
       const int key = (pos[q] << 8) | Rows<N, G, R>::row(ln, q);
       const bool better = act && (v > bv || (v == bv && key < bkey));
       bv = better ? v : bv;
@@ -246,35 +324,35 @@ __device__ __forceinline__ void lu_real(double (&a)[R][N], int (&pos)[R], Perm<N
     const int plane = prow % G;
     const int pq = prow / G;
     perm.set(k, prow);
-    double mine[R];
+    T mine[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) mine[q] = a[q][k];
-    const double piv = Grp<G>::bcast(sel<R>(mine, pq), plane);
+    const T piv = Grp<G>::bcast(sel<R>(mine, pq), plane);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (pos[q] == k) pos[q] = ppos;  // the row LAPACK swaps out of position k
-      if (Rows<N, G, R>::row(ln, q) == prow) pos[q] = k;
+      const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
+      pos[q] = is_piv ? k : (pos[q] == k ? ppos : pos[q]);  // LAPACK swaps rows k and ppos
     }
-    const bool nz = piv != 0.0;
+    const bool nz = piv != T(0);
     if (!nz && info == 0) info = k + 1;
-    const bool tiny = fabs(piv) < kDblMin;
-    const double rp = 1.0 / piv;
-    double l[R];
+    const bool tiny = fabs(piv) < min_normal<T>();
+    const T rp = T(1) / piv;
+    T l[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (Rows<N, G, R>::row(ln, q) == prow) rd[q] = nz ? rp : 0.0;
+      rd[q] = (Rows<N, G, R>::row(ln, q) == prow) ? (nz ? rp : T(0)) : rd[q];
       const bool act = nz && pos[q] > k;
-      const double lq = tiny ? a[q][k] / piv : a[q][k] * rp;
-      l[q] = act ? lq : 0.0;
-      if (act) a[q][k] = lq;
+      const T lq = tiny ? a[q][k] / piv : a[q][k] * rp;
+      l[q] = act ? lq : T(0);
+      a[q][k] = act ? lq : a[q][k];
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (j > k) {
-        double m2[R];
+        T m2[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) m2[q] = a[q][j];
-        const double uj = Grp<G>::bcast(sel<R>(m2, pq), plane);
+        const T uj = Grp<G>::bcast(sel<R>(m2, pq), plane);
 #pragma unroll
         for (int q = 0; q < R; ++q) a[q][j] = fma(-l[q], uj, a[q][j]);
       }
@@ -283,26 +361,26 @@ __device__ __forceinline__ void lu_real(double (&a)[R][N], int (&pos)[R], Perm<N
 }
 
 // ---- complex LU (zgetf2 semantics) -------------------------------------------
-template <int N, int G, int R>
-__device__ __forceinline__ void lu_cplx(double (&ar)[R][N], double (&ai)[R][N], int (&pos)[R],
-                                        Perm<N>& perm, double (&rdr)[R], double (&rdi)[R],
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void lu_cplx(T (&ar)[R][N], T (&ai)[R][N], int (&pos)[R],
+                                        Perm<N>& perm, T (&rdr)[R], T (&rdi)[R],
                                         int& info) {
   const int ln = Grp<G>::lane();
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     pos[q] = Rows<N, G, R>::valid(ln, q) ? Rows<N, G, R>::row(ln, q) : -1;
-    rdr[q] = 0.0;
-    rdi[q] = 0.0;
+    rdr[q] = T(0);
+    rdi[q] = T(0);
   }
   info = 0;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    double bv = -1.0;
+    T bv = T(-1);
     int bkey = 0x7fffffff;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const bool act = pos[q] >= k;
-      const double v = fabs(ar[q][k]) + fabs(ai[q][k]);
+      const T v = fabs(ar[q][k]) + fabs(ai[q][k]);
       const int key = (pos[q] << 8) | Rows<N, G, R>::row(ln, q);
       const bool better = act && (v > bv || (v == bv && key < bkey));
       bv = better ? v : bv;
@@ -314,43 +392,43 @@ __device__ __forceinline__ void lu_cplx(double (&ar)[R][N], double (&ai)[R][N], 
     const int plane = prow % G;
     const int pq = prow / G;
     perm.set(k, prow);
-    double mr[R], mi[R];
+    T mr[R], mi[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) { mr[q] = ar[q][k]; mi[q] = ai[q][k]; }
-    const double pr_ = Grp<G>::bcast(sel<R>(mr, pq), plane);
-    const double pi_ = Grp<G>::bcast(sel<R>(mi, pq), plane);
+    const T pr_ = Grp<G>::bcast(sel<R>(mr, pq), plane);
+    const T pi_ = Grp<G>::bcast(sel<R>(mi, pq), plane);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (pos[q] == k) pos[q] = ppos;
-      if (Rows<N, G, R>::row(ln, q) == prow) pos[q] = k;
+      const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
+      pos[q] = is_piv ? k : (pos[q] == k ? ppos : pos[q]);
     }
-    const bool nz = (pr_ != 0.0 || pi_ != 0.0);
+    const bool nz = (pr_ != T(0) || pi_ != T(0));
     if (!nz && info == 0) info = k + 1;
-    double rr, ri;
+    T rr, ri;
     crecip(pr_, pi_, rr, ri);
-    double lr[R], li[R];
+    T lr[R], li[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      if (Rows<N, G, R>::row(ln, q) == prow) {
-        rdr[q] = nz ? rr : 0.0;
-        rdi[q] = nz ? ri : 0.0;
-      }
+      const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
+      rdr[q] = is_piv ? (nz ? rr : T(0)) : rdr[q];
+      rdi[q] = is_piv ? (nz ? ri : T(0)) : rdi[q];
       const bool act = nz && pos[q] > k;
-      const double xr = ar[q][k], xi = ai[q][k];
-      const double vr = xr * rr - xi * ri;
-      const double vi = xr * ri + xi * rr;
-      lr[q] = act ? vr : 0.0;
-      li[q] = act ? vi : 0.0;
-      if (act) { ar[q][k] = vr; ai[q][k] = vi; }
+      const T xr = ar[q][k], xi = ai[q][k];
+      const T vr = xr * rr - xi * ri;
+      const T vi = xr * ri + xi * rr;
+      lr[q] = act ? vr : T(0);
+      li[q] = act ? vi : T(0);
+      ar[q][k] = act ? vr : ar[q][k];
+      ai[q][k] = act ? vi : ai[q][k];
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (j > k) {
-        double m2r[R], m2i[R];
+        T m2r[R], m2i[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) { m2r[q] = ar[q][j]; m2i[q] = ai[q][j]; }
-        const double ur = Grp<G>::bcast(sel<R>(m2r, pq), plane);
-        const double ui = Grp<G>::bcast(sel<R>(m2i, pq), plane);
+        const T ur = Grp<G>::bcast(sel<R>(m2r, pq), plane);
+        const T ui = Grp<G>::bcast(sel<R>(m2i, pq), plane);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           ar[q][j] = fma(-lr[q], ur, ar[q][j]);
@@ -368,78 +446,80 @@ __device__ __forceinline__ void lu_cplx(double (&ar)[R][N], double (&ai)[R][N], 
 // solution component r.  Forward then backward column-oriented substitution in
 // pivot order, as reference BLAS dtrsm does it; rows outside the current
 // step's range take a zero coefficient.
-template <int N, int G, int R>
-__device__ __forceinline__ void solve_real(const double (&a)[R][N], const int (&pos)[R],
-                                           const Perm<N>& perm, const double (&rd)[R],
-                                           double (&b)[R]) {
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void solve_real(const T (&a)[R][N], const int (&pos)[R],
+                                           const Perm<N>& perm, const T (&rd)[R],
+                                           T (&b)[R]) {
   const int ln = Grp<G>::lane();
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int prow = perm.get(k);
-    const double yk = Grp<G>::bcast(sel<R>(b, prow / G), prow % G);
+    const T yk = Grp<G>::bcast(sel<R>(b, prow / G), prow % G);
 #pragma unroll
-    for (int q = 0; q < R; ++q) b[q] = fma(-(pos[q] > k ? a[q][k] : 0.0), yk, b[q]);
+    for (int q = 0; q < R; ++q) b[q] = fma(-(pos[q] > k ? a[q][k] : T(0)), yk, b[q]);
   }
-  double x[R];
+  T x[R];
 #pragma unroll
-  for (int q = 0; q < R; ++q) x[q] = 0.0;
+  for (int q = 0; q < R; ++q) x[q] = T(0);
 #pragma unroll
   for (int k = N - 1; k >= 0; --k) {
     const int prow = perm.get(k);
     const int pq = prow / G;
-    const double xk = Grp<G>::bcast(sel<R>(b, pq) * sel<R>(rd, pq), prow % G);
+    const T xk = Grp<G>::bcast(sel<R>(b, pq) * sel<R>(rd, pq), prow % G);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
-      b[q] = fma(-((pos[q] >= 0 && pos[q] < k) ? a[q][k] : 0.0), xk, b[q]);
-      if (Rows<N, G, R>::row(ln, q) == k) x[q] = xk;
+      b[q] = fma(-((pos[q] >= 0 && pos[q] < k) ? a[q][k] : T(0)), xk, b[q]);
+      x[q] = (Rows<N, G, R>::row(ln, q) == k) ? xk : x[q];
     }
   }
 #pragma unroll
   for (int q = 0; q < R; ++q) b[q] = x[q];
 }
 
-template <int N, int G, int R>
-__device__ __forceinline__ void solve_cplx(const double (&ar)[R][N], const double (&ai)[R][N],
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void solve_cplx(const T (&ar)[R][N], const T (&ai)[R][N],
                                            const int (&pos)[R], const Perm<N>& perm,
-                                           const double (&rdr)[R], const double (&rdi)[R],
-                                           double (&br)[R], double (&bi)[R]) {
+                                           const T (&rdr)[R], const T (&rdi)[R],
+                                           T (&br)[R], T (&bi)[R]) {
   const int ln = Grp<G>::lane();
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int prow = perm.get(k);
     const int pl = prow % G, pq = prow / G;
-    const double yr = Grp<G>::bcast(sel<R>(br, pq), pl);
-    const double yi = Grp<G>::bcast(sel<R>(bi, pq), pl);
+    const T yr = Grp<G>::bcast(sel<R>(br, pq), pl);
+    const T yi = Grp<G>::bcast(sel<R>(bi, pq), pl);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const bool act = pos[q] > k;
-      const double lr = act ? ar[q][k] : 0.0, li = act ? ai[q][k] : 0.0;
+      const T lr = act ? ar[q][k] : T(0), li = act ? ai[q][k] : T(0);
       br[q] = fma(-lr, yr, br[q]);
       br[q] = fma(li, yi, br[q]);
       bi[q] = fma(-lr, yi, bi[q]);
       bi[q] = fma(-li, yr, bi[q]);
     }
   }
-  double xr[R], xi[R];
+  T xr[R], xi[R];
 #pragma unroll
-  for (int q = 0; q < R; ++q) { xr[q] = 0.0; xi[q] = 0.0; }
+  for (int q = 0; q < R; ++q) { xr[q] = T(0); xi[q] = T(0); }
 #pragma unroll
   for (int k = N - 1; k >= 0; --k) {
     const int prow = perm.get(k);
     const int pl = prow % G, pq = prow / G;
-    const double vr = sel<R>(br, pq), vi = sel<R>(bi, pq);
-    const double dr = sel<R>(rdr, pq), di = sel<R>(rdi, pq);
-    const double kr = Grp<G>::bcast(vr * dr - vi * di, pl);
-    const double ki = Grp<G>::bcast(vr * di + vi * dr, pl);
+    const T vr = sel<R>(br, pq), vi = sel<R>(bi, pq);
+    const T dr = sel<R>(rdr, pq), di = sel<R>(rdi, pq);
+    const T kr = Grp<G>::bcast(vr * dr - vi * di, pl);
+    const T ki = Grp<G>::bcast(vr * di + vi * dr, pl);
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const bool act = pos[q] >= 0 && pos[q] < k;
-      const double ur = act ? ar[q][k] : 0.0, ui = act ? ai[q][k] : 0.0;
+      const T ur = act ? ar[q][k] : T(0), ui = act ? ai[q][k] : T(0);
       br[q] = fma(-ur, kr, br[q]);
       br[q] = fma(ui, ki, br[q]);
       bi[q] = fma(-ur, ki, bi[q]);
       bi[q] = fma(-ui, kr, bi[q]);
-      if (Rows<N, G, R>::row(ln, q) == k) { xr[q] = kr; xi[q] = ki; }
+      const bool mine = Rows<N, G, R>::row(ln, q) == k;
+      xr[q] = mine ? kr : xr[q];
+      xi[q] = mine ? ki : xi[q];
     }
   }
 #pragma unroll
@@ -735,11 +815,13 @@ __device__ __forceinline__ void rsolve_cplx(const double (&ar)[N], const double 
 //   static constexpr int N;
 //   struct Ctx;  (per-group context: LDS pointers, tables)
 //   template <int G, int R> dydt(Ctx&, t, pr, const double (&y)[R], double (&f)[R])
-//   template <int G, int R> jacob(Ctx&, t, pr, const double (&y)[R], double (&J)[R][N])
 //   static constexpr bool kJacInLds;  double* Ctx::jac  (N*N doubles when true)
-// When kJacInLds is set the Jacobian is parked in the group's LDS between
-// evaluations (it is only read when the iteration matrices are rebuilt),
-// which keeps N*R doubles per lane out of the register file.
+//   kJacInLds:  template <int G, int R> jacob(Ctx&, t, pr, const double (&y)[R])
+//               writing rows ctx.jac[r*N + j] = d f_r / d y_j
+//   otherwise:  template <int G, int R> jacob(Ctx&, t, pr, const double (&y)[R], double (&J)[R][N])
+// When kJacInLds is set the Jacobian lives in the group's LDS (it is only read
+// when the iteration matrices are rebuilt), which keeps N*R doubles per lane
+// out of the register file.
 //
 // The reference's integrate() (radau2a.c:520-736) is a loop over step
 // attempts.  Here it is a resumable state machine: begin() is the prologue
@@ -749,16 +831,24 @@ __device__ __forceinline__ void rsolve_cplx(const double (&ar)[N], const double 
 // new condition or a new global step as soon as its current one ends.
 enum : int { RADAU_CONTINUE = -1 };
 
-template <class Mech, int G, int R>
+template <class Mech, int G, int R, class MT = double>
 struct Radau {
   static constexpr int N = Mech::N;
   using RW = Rows<N, G, R>;
   using Ctx = typename Mech::Ctx;
+  // Matrix scalar type: double, or float for the mixed-precision method
+  // (iteration-matrix factorizations and Newton solves in FP32; residuals,
+  // error estimate, step control and state in FP64).
+  static constexpr bool kMixed = std::is_same<MT, float>::value;
+  static_assert(kMixed || std::is_same<MT, double>::value, "matrix type must be float or double");
   // Column-layout LU with compile-time broadcast lanes when one lane per
-  // column fits in a DPP row (G = 16) or a wave (G = 64) and the Jacobian is
-  // parked in LDS (it is read back column-wise); implicit-pivot row LU
-  // otherwise.
-  static constexpr bool kColLU = (G == 16 || G == 64) && G >= N && R == 1 && Mech::kJacInLds;
+  // column fits in a DPP row (G = 16) and the Jacobian is parked in LDS (it is
+  // read back column-wise); implicit-pivot row LU otherwise.  At G = 64 the
+  // row LU broadcasts pivot rows through SGPRs (readlane), which is cheaper
+  // than the column form's interchange selects.
+  static constexpr bool kColLU = G == 16 && G >= N && R == 1 && Mech::kJacInLds && !kMixed;
+  // The column LU transposes its factors through a per-group LDS scratch.
+  static constexpr bool kNeedsScratch = kColLU;
 
   // per-interval controller state (group-uniform)
   double t, t_end, H, Hold, Hacc, ErrOld, NewtonRate;
@@ -768,28 +858,87 @@ struct Radau {
   double y[R], sc[R], y0[R], F0[R], Z1[R], Z2[R], Z3[R];
   double CONT0[R], CONT1[R], CONT2[R];
   // factorizations
-  double E1[R][N], E2r[R][N], E2i[R][N];
+  MT E1[R][N], E2r[R][N], E2i[R][N];
   int pos1[R], pos2[R];
-  double rd1[R], rd2r[R], rd2i[R];
+  MT rd1[R], rd2r[R], rd2i[R];
   Perm<N> perm1, perm2;
   int src1, src2;  // column-LU path: original equation of this lane's row
   double A[R][N];  // Jacobian (only live across attempts when !Mech::kJacInLds)
 
   __device__ __forceinline__ void solve1(double (&b)[R]) const {
-    if constexpr (kColLU) rsolve_real<N, G>(E1[0], src1, rd1[0], b[0]);
-    else solve_real<N, G, R>(E1, pos1, perm1, rd1, b);
+    if constexpr (kColLU) {
+      rsolve_real<N, G>(E1[0], src1, rd1[0], b[0]);
+    } else if constexpr (!kMixed) {
+      solve_real<N, G, R, double>(E1, pos1, perm1, rd1, b);
+    } else {
+      MT bb[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) bb[q] = (MT)b[q];
+      solve_real<N, G, R, MT>(E1, pos1, perm1, rd1, bb);
+#pragma unroll
+      for (int q = 0; q < R; ++q) b[q] = (double)bb[q];
+    }
   }
   __device__ __forceinline__ void solve2(double (&br)[R], double (&bi)[R]) const {
-    if constexpr (kColLU) rsolve_cplx<N, G>(E2r[0], E2i[0], src2, rd2r[0], rd2i[0], br[0], bi[0]);
-    else solve_cplx<N, G, R>(E2r, E2i, pos2, perm2, rd2r, rd2i, br, bi);
+    if constexpr (kColLU) {
+      rsolve_cplx<N, G>(E2r[0], E2i[0], src2, rd2r[0], rd2i[0], br[0], bi[0]);
+    } else if constexpr (!kMixed) {
+      solve_cplx<N, G, R, double>(E2r, E2i, pos2, perm2, rd2r, rd2i, br, bi);
+    } else {
+      MT xr[R], xi[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) { xr[q] = (MT)br[q]; xi[q] = (MT)bi[q]; }
+      solve_cplx<N, G, R, MT>(E2r, E2i, pos2, perm2, rd2r, rd2i, xr, xi);
+#pragma unroll
+      for (int q = 0; q < R; ++q) { br[q] = (double)xr[q]; bi[q] = (double)xi[q]; }
+    }
+  }
+
+  // Error-estimate solve.  FP64 factors: one solve.  FP32 factors: one solve
+  // plus one step of iterative refinement against the FP64 matrix
+  // gamma/H - J (J = the Jacobian the factors were built from), so the
+  // estimate keeps FP64 accuracy whenever cond(E1) * 2^-24 << 1.
+  __device__ __forceinline__ void solve1_es(Ctx& ctx, double (&b)[R]) const {
+    if constexpr (!kMixed) {
+      solve1(b);
+    } else {
+      const int ln = Grp<G>::lane();
+      double x[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) x[q] = b[q];
+      solve1(x);
+      if constexpr (Mech::kJacInLds) group_lds_sync();
+      const double g1 = rkGamma / H;
+      double r[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) r[q] = b[q] - g1 * x[q];
+      static_for<0, N>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const double xj = bcast_k<G, j % G>(x[j / G]);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          double Jrj;
+          if constexpr (Mech::kJacInLds) Jrj = RW::valid(ln, q) ? ctx.jac[RW::row(ln, q) * N + j] : 0.0;
+          else Jrj = A[q][j];
+          r[q] = fma(Jrj, xj, r[q]);
+        }
+      });
+#pragma unroll
+      for (int q = 0; q < R; ++q) r[q] = RW::valid(ln, q) ? r[q] : 0.0;
+      solve1(r);
+#pragma unroll
+      for (int q = 0; q < R; ++q) b[q] = x[q] + r[q];
+    }
   }
 
   // RK_Decomp, radau2a.c:256-278: E1 = gamma/H - J, E2 = (alpha + i beta)/H - J,
-  // then the real and (if E1 is regular) the complex LU.  Returns info.
+  // then the real and (if E1 is regular) the complex LU.  Returns info.  The
+  // complex factorization also runs when E1 is singular (its info is then
+  // ignored, as the reference skips it): that keeps E2 defined on every path.
   __device__ __forceinline__ int decomp(Ctx& ctx) {
     const int ln = Grp<G>::lane();
     const double g1 = rkGamma / H, g2r = rkAlpha / H, g2i = rkBeta / H;
-    int info = 0;
+    int info = 0, info2 = 0;
     if constexpr (kColLU) {
       group_lds_sync();  // the parked Jacobian rows of all lanes
       double c1[N], c2r[N], c2i[N];
@@ -805,28 +954,30 @@ struct Radau {
       clu_real<N, G>(c1, m1, rd1[0], info);
       transpose_cols_to_rows<N, G>(c1, E1[0], ctx.scratch);
       src1 = m1.get(ln);
-      if (info == 0) {
-        clu_cplx<N, G>(c2r, c2i, m2, rd2r[0], rd2i[0], info);
-        transpose_cols_to_rows<N, G>(c2r, E2r[0], ctx.scratch);
-        transpose_cols_to_rows<N, G>(c2i, E2i[0], ctx.scratch);
-        src2 = m2.get(ln);
-      }
+      clu_cplx<N, G>(c2r, c2i, m2, rd2r[0], rd2i[0], info2);
+      transpose_cols_to_rows<N, G>(c2r, E2r[0], ctx.scratch);
+      transpose_cols_to_rows<N, G>(c2i, E2i[0], ctx.scratch);
+      src2 = m2.get(ln);
     } else {
+      if constexpr (Mech::kJacInLds) group_lds_sync();
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int r = RW::row(ln, q);
 #pragma unroll
         for (int j = 0; j < N; ++j) {
           const bool d = (j == r);
-          E1[q][j] = -A[q][j] + (d ? g1 : 0.0);
-          E2r[q][j] = -A[q][j] + (d ? g2r : 0.0);
-          E2i[q][j] = d ? g2i : 0.0;
+          double aj;
+          if constexpr (Mech::kJacInLds) aj = RW::valid(ln, q) ? ctx.jac[r * N + j] : 0.0;
+          else aj = A[q][j];
+          E1[q][j] = (MT)(-aj + (d ? g1 : 0.0));
+          E2r[q][j] = (MT)(-aj + (d ? g2r : 0.0));
+          E2i[q][j] = (MT)(d ? g2i : 0.0);
         }
       }
-      lu_real<N, G, R>(E1, pos1, perm1, rd1, info);
-      if (info == 0) lu_cplx<N, G, R>(E2r, E2i, pos2, perm2, rd2r, rd2i, info);
+      lu_real<N, G, R, MT>(E1, pos1, perm1, rd1, info);
+      lu_cplx<N, G, R, MT>(E2r, E2i, pos2, perm2, rd2r, rd2i, info2);
     }
-    return info;
+    return info != 0 ? info : info2;
   }
 
   // RK_ErrorNorm, radau2a.c:446-453: max(sqrt(sum (sc d)^2 / N), 1e-10)
@@ -872,22 +1023,9 @@ struct Radau {
     }
     if (!SkipLU) {
       if (!SkipJac) {
-        Mech::template jacob<G, R>(ctx, t, pr, y, A);
+        if constexpr (Mech::kJacInLds) Mech::template jacob<G, R>(ctx, t, pr, y);
+        else Mech::template jacob<G, R>(ctx, t, pr, y, A);
         cnt[CNT_JAC]++;
-        if constexpr (Mech::kJacInLds) {
-#pragma unroll
-          for (int q = 0; q < R; ++q)
-            if (RW::valid(ln, q)) {
-#pragma unroll
-              for (int j = 0; j < N; ++j) ctx.jac[RW::row(ln, q) * N + j] = A[q][j];
-            }
-        }
-      } else if constexpr (Mech::kJacInLds && !kColLU) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-#pragma unroll
-          for (int j = 0; j < N; ++j) A[q][j] = RW::valid(ln, q) ? ctx.jac[RW::row(ln, q) * N + j] : 0.0;
-        }
       }
       const int info = decomp(ctx);
       cnt[CNT_LU]++;
@@ -1025,7 +1163,7 @@ struct Radau {
       }
 #pragma unroll 1
       for (int pass = 0;; ++pass) {
-        solve1(Tm);
+        solve1_es(ctx, Tm);
         cnt[CNT_ES]++;
         Err = err_norm(Tm);
         if (pass == 1 || !(Err >= 1.0 && (FirstStep || Reject))) break;

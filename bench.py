@@ -33,7 +33,10 @@ CONFIGS = {
     "c0": (2, 4096, 1e-3, None, "van der Pol, mu=10^U(1,3), 4096 conditions, dt=1e-3"),
     "c1": (14, 262144, 1e-6, (1000.0, 1600.0),
            "synthetic kinetics 14 states (13 species + T), 27 reactions, 262144 conditions, dt=1e-6 s"),
+    "c2": (54, 131072, 1e-6, (800.0, 2200.0),
+           "synthetic kinetics 54 states (53 species + T), 325 reactions, 131072 conditions, dt=1e-6 s"),
 }
+DEFAULT_PRECISION = {"c0": "fp64", "c1": "fp64", "c2": "mixed"}
 
 
 def log(*a):
@@ -104,6 +107,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
     ap.add_argument("--num", type=int, default=0, help="override condition count")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
+                    help="iteration-matrix arithmetic (default: mixed for c2, fp64 otherwise)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain-FP64 comparison leg of a mixed-precision run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (rocprof traffic calibration)")
@@ -138,40 +145,62 @@ def main():
     acc.initialize(num, 0)
 
     dev = torch.device("cuda:0")
-    y_dev = torch.from_numpy(np.ascontiguousarray(y0.T)).to(dev)  # condition-major
-    pr_dev = torch.from_numpy(pr).to(dev)
-    codes_dev = torch.zeros(num, dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
     # one explicit (default-priority) stream for the library's launches and the
     # timing events, so the events bracket exactly the integration kernels
     stream = torch.cuda.Stream(device=dev)
     sp = stream.cuda_stream
     assert sp != 0
+    pr_dev = torch.from_numpy(pr).to(dev)
+    codes_dev = torch.zeros(num, dtype=torch.int32, device=dev)
 
-    # warmup: the first W global steps of the workload, one launch, untimed
-    t = 0.0
-    if args.warmup:
-        acc.integrate_device(num, t, t + args.warmup * dt, dt, y_dev.data_ptr(), pr_dev.data_ptr(),
+    def run_leg(precision):
+        """Warmup W global steps (untimed), then K timed global steps in one
+        persistent launch, from the same initial states."""
+        acc.set_precision(acc.PRECISION_MIXED if precision == "mixed" else acc.PRECISION_FP64)
+        y_dev = torch.from_numpy(np.ascontiguousarray(y0.T)).to(dev)  # condition-major
+        torch.cuda.synchronize()
+        t = 0.0
+        if args.warmup:
+            acc.integrate_device(num, t, t + args.warmup * dt, dt, y_dev.data_ptr(), pr_dev.data_ptr(),
+                                 codes_dev.data_ptr(), sp)
+            t += args.warmup * dt
+        torch.cuda.synchronize()
+        acc.reset_counters()
+        torch.cuda.synchronize()
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(stream)
+        acc.integrate_device(num, t, t + args.steps * dt, dt, y_dev.data_ptr(), pr_dev.data_ptr(),
                              codes_dev.data_ptr(), sp)
-        t += args.warmup * dt
-    torch.cuda.synchronize()
-    acc.reset_counters()
-    torch.cuda.synchronize()
+        end.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = start.elapsed_time(end)
+        codes = codes_dev.cpu().numpy()
+        yf = y_dev.cpu().numpy().T
+        cnt = acc.get_counters(num)
+        flops = algorithmic_flops(cnt, n, F, Jc)
+        per_unit = cnt.sum(0) / (num * args.steps)
+        log("[%s] per condition-step events: " % precision
+            + ", ".join("%s=%.2f" % (k, v) for k, v in zip(acc.CNT_NAMES, per_unit)))
+        log("[%s] integration launch %.3f ms (%d global steps); wall %.3f s; nonzero codes %d; finite %s; "
+            "algorithmic FLOPs per condition-step %.3g" % (
+                precision, ms, args.steps, wall, int((codes != 0).sum()), bool(np.isfinite(yf).all()),
+                flops / (num * args.steps)))
+        return {"wall": wall, "ms": ms, "codes": codes, "y": yf, "flops": flops}
 
-    # timed: the next K global steps, one persistent launch (every condition runs
-    # its K steps back to back), bracketed by events on the launch stream
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    acc.integrate_device(num, t, t + args.steps * dt, dt, y_dev.data_ptr(), pr_dev.data_ptr(),
-                         codes_dev.data_ptr(), sp)
-    end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    launch_ms = [start.elapsed_time(end)]
-    avg_ms = float(np.mean(launch_ms))
-    t += args.steps * dt
+    precision = args.precision or DEFAULT_PRECISION[args.config]
+    log("config %s: %s" % (args.config, desc))
+    main_leg = run_leg(precision)
+    compare = None
+    if precision == "mixed" and not args.no_compare:
+        ref = run_leg("fp64")
+        diff = np.abs(main_leg["y"] - ref["y"]) / (1e-10 + 1e-6 * np.abs(ref["y"]))
+        compare = {"plain_fp64_value": num * args.steps / ref["wall"],
+                   "speedup_vs_plain_fp64": ref["wall"] / main_leg["wall"],
+                   "max_weighted_diff_vs_fp64": float(diff.max()),
+                   "plain_fp64_nonzero_codes": int((ref["codes"] != 0).sum())}
+        log("compare: %s" % json.dumps(compare))
 
     if args.calib_copy_mb:
         # a plain copy of known size, for scaling rocprof FETCH_SIZE/WRITE_SIZE
@@ -181,27 +210,22 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mb, args.calib_copy_mb))
-    codes = codes_dev.cpu().numpy()
-    yf = y_dev.cpu().numpy()
-    cnt = acc.get_counters(num)
-    flops_total = algorithmic_flops(cnt, n, F, Jc)
-    flops_per_launch = flops_total / len(launch_ms)
-    achieved_tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    per_unit = cnt.sum(0) / (num * args.steps)
-    log("config %s: %s" % (args.config, desc))
-    log("per condition-step events: " + ", ".join("%s=%.2f" % (k, v) for k, v in zip(acc.CNT_NAMES, per_unit)))
-    log("integration launch %.3f ms (%d global steps); wall %.3f s; nonzero codes %d; finite %s" % (
-        avg_ms, args.steps, wall, int((codes != 0).sum()), bool(np.isfinite(yf).all())))
-    log("algorithmic FLOPs per condition-step %.3g" % (flops_total / (num * args.steps)))
 
+    achieved_tflops = main_leg["flops"] / (main_leg["ms"] * 1e-3) / 1e12
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, mech, y0, pr, dt)
         log("cpu baseline: %s" % json.dumps(cpu))
-    value = num * args.steps / wall
+    wall = main_leg["wall"]
+    method = ("mixed-precision Radau IIA-5 (FP32 iteration-matrix LU and Newton solves, FP64 residuals, "
+              "error estimate, step control, state)" if precision == "mixed" else "plain FP64 Radau IIA-5")
+    config = {"workload": "%s: %s" % (args.config, desc), "method": method,
+              "rtol": 1e-6, "atol": 1e-10, "n_states": n, "conditions": num, "dt": dt}
+    if compare:
+        config.update(compare)
     out = {
         "metric": "conditions/sec per 1e-6 s step" if n != 2 else "conditions/sec per 1e-3 s step",
-        "value": value,
+        "value": num * args.steps / wall,
         "unit": "conditions*steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -210,15 +234,17 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if precision == "fp64" else "f64+f32",
         "data": "synthetic",
-        "config": {"workload": "%s: %s" % (args.config, desc), "method": "plain FP64 Radau IIA-5",
-                   "rtol": 1e-6, "atol": 1e-10, "n_states": n, "conditions": num, "dt": dt},
+        "config": config,
+        # algorithmic FLOPs (both precisions) over the FP64 vector peak: FP32
+        # non-packed VALU issues at the same rate as FP64 on this chip
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    acc.set_precision(acc.PRECISION_FP64)
     acc.cleanup()
 
 

@@ -86,6 +86,15 @@ int accelerInt_set_kinetics(int ns, int nr, const double *W, const double *therm
 int accelerInt_set_tolerances(double atol, double rtol);
 int accelerInt_set_const_time_step(int on);
 
+/* Arithmetic of the iteration matrices.  FP64 (default): everything in double,
+ * as the reference.  MIXED: the real and complex LU factorizations and the
+ * Newton solves in float; dydt residuals, the error estimate (its solve gets one
+ * FP64 iterative-refinement step), step control and the state stay double.
+ * May be changed between integrate calls.  Returns 0, or -1 for an unknown mode. */
+#define ACCELERINT_PRECISION_FP64 0
+#define ACCELERINT_PRECISION_MIXED 1
+int accelerInt_set_precision(int mode);
+
 /* 1 (default): any non-zero condition code aborts the process through
  * check_error, as the reference does.  0: codes are kept and returned by
  * accelerInt_get_status; integration continues for the other conditions. */

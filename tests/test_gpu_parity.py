@@ -138,3 +138,87 @@ def test_kinetics14_parity(session, oracle_lib, net14):
     assert we.max() <= 10.0
     cnt_g = acc.get_counters(num).astype(np.int64)
     print("conditions with identical event counters: %.3f" % (cnt_g == cnt_o).all(1).mean())
+
+
+def _kin_parity(oracle_lib, mech, num, steps, dt, seed, T_range, precision, label, min_same=None):
+    y0, P = M.initial_conditions(mech, num, seed=seed, T_range=T_range)
+    _kin_setup(mech, num)
+    acc.set_precision(precision)
+    try:
+        y = y0.copy()
+        acc.integrate(num, 0.0, steps * dt, dt, y, P)
+        codes = acc.get_status(num)
+        cnt_g = acc.get_counters(num).astype(np.int64)
+    finally:
+        acc.set_precision(acc.PRECISION_FP64)
+    om = oracle_lib.Mechanism.kinetics(mech)
+    yo, co, cnt_o = oracle_lib.integrate_steps(om, y0, P, dt, steps, nthreads=8)
+    assert (codes == co).all()
+    assert np.isfinite(y).all()
+    we = weighted_err(y, yo)
+    rel = np.abs(y - yo) / np.maximum(np.abs(yo), 1e-300)
+    same = (cnt_g == cnt_o).all(1).mean()
+    print("%s weighted err median %.3g p99 %.3g max %.3g; max rel err (|y| > atol) %.3g; identical counters %.3f"
+          % (label, np.median(we), np.percentile(we, 99), we.max(), rel[np.abs(yo) > ATOL].max(), same))
+    assert we.max() <= 10.0
+    if min_same is not None:
+        assert same >= min_same
+    return cnt_g, cnt_o
+
+
+def test_kinetics14_mixed_parity(session, oracle_lib, net14):
+    """Mixed precision (FP32 LU/solves) on the 14-state network against the FP64 oracle."""
+    _kin_parity(oracle_lib, net14, 1024, 20, 1e-6, 20261202, (1000.0, 1600.0), acc.PRECISION_MIXED,
+                "kinetics14 mixed")
+
+
+@pytest.fixture(scope="module")
+def net54():
+    return M.network(54)
+
+
+def test_kinetics54_callbacks(session, oracle_lib, net54):
+    num = 256
+    y0, P = M.initial_conditions(net54, num, seed=12, T_range=(800.0, 2200.0))
+    _kin_setup(net54, num)
+    dy, jac = acc.eval_callbacks(num, 0.0, y0, P)
+    om = oracle_lib.Mechanism.kinetics(net54)
+    for c in range(0, num, 23):
+        fo = oracle_lib.dydt(om, 0.0, P[c], y0[:, c])
+        Jo = oracle_lib.jacobian(om, 0.0, P[c], y0[:, c])
+        assert np.allclose(dy[:, c], fo, rtol=1e-11, atol=1e-12 * np.abs(fo).max())
+        scale = np.abs(Jo).max(1, keepdims=True)
+        assert (np.abs(jac[c] - Jo) <= 1e-11 * scale).all()
+
+
+def test_kinetics54_parity(session, oracle_lib, net54):
+    """54-state network, plain FP64, 512 conditions, 5 global steps of 1e-6 s."""
+    _kin_parity(oracle_lib, net54, 512, 5, 1e-6, 20261204, (800.0, 2200.0), acc.PRECISION_FP64,
+                "kinetics54 fp64", min_same=0.5)
+
+
+def test_kinetics54_mixed_parity(session, oracle_lib, net54):
+    """54-state network, FP32 iteration-matrix LU and solves, against the FP64 oracle."""
+    _kin_parity(oracle_lib, net54, 512, 5, 1e-6, 20261204, (800.0, 2200.0), acc.PRECISION_MIXED,
+                "kinetics54 mixed")
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_kinetics54_tight_golden(session, net54, precision):
+    """Both precisions against scipy BDF at rtol 1e-11 (tests/golden/kinetics54_tight.npz)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kinetics54_tight.npz"))
+    num = g["y0"].shape[1]
+    _kin_setup(net54, num)
+    acc.set_precision(precision)
+    try:
+        y = np.ascontiguousarray(g["y0"]).copy()
+        acc.integrate(num, 0.0, int(g["steps"]) * float(g["dt"]), float(g["dt"]), y, np.ascontiguousarray(g["P"]))
+        codes = acc.get_status(num)
+    finally:
+        acc.set_precision(acc.PRECISION_FP64)
+    assert (codes == 0).all()
+    err = weighted_err(y, g["y_tight"])
+    print("kinetics54 precision %d vs tight: median %.3g max %.3g" % (precision, np.median(err), err.max()))
+    assert np.median(err) <= 0.5
+    assert err.max() <= 10.0

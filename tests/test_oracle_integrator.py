@@ -109,3 +109,18 @@ def test_error_codes(oracle_lib):
     m = oracle_lib.Mechanism.linear([[1.0]])
     y, codes, cnt = oracle_lib.integrate(m, 1e10, 1e10 + 1e-6, np.array([1e8]), np.array([[1.0]]))
     assert codes[0] == 3  # 0.1 |H| <= |t| eps at t = 1e10
+
+
+@pytest.mark.parametrize("n", [14, 54])
+def test_kinetics_against_tight_golden(oracle_lib, n):
+    """Synthetic kinetics networks against scipy BDF at rtol 1e-11
+    (tests/golden/kinetics{14,54}_tight.npz, written by make_golden.py)."""
+    from accelerint_amd import mechanism as M
+    g = np.load(os.path.join(GOLDEN, "kinetics%d_tight.npz" % n))
+    om = oracle_lib.Mechanism.kinetics(M.network(n))
+    y, codes, _ = oracle_lib.integrate_steps(om, g["y0"], g["P"], float(g["dt"]), int(g["steps"]))
+    assert (codes == 0).all()
+    err = np.abs(y - g["y_tight"]) / (1e-10 + 1e-6 * np.abs(g["y_tight"]))
+    print("kinetics%d vs tight: median %.3g max %.3g" % (n, np.median(err), err.max()))
+    assert np.median(err) <= 0.5
+    assert err.max() <= 10.0
