@@ -39,6 +39,7 @@ EXPORTS = (
     "accelerInt_get_status", "accelerInt_get_counters", "accelerInt_reset_counters",
     "accelerInt_num_states", "accelerInt_integrate_device", "accelerInt_synchronize",
     "accelerInt_last_error", "accelerInt_eval_callbacks", "accelerInt_set_precision",
+    "accelerInt_escalated",
 )
 
 
@@ -83,6 +84,7 @@ def lib():
         "accelerInt_last_error": ([], ctypes.c_char_p),
         "accelerInt_eval_callbacks": ([i, d, v, v, v, v], i),
         "accelerInt_set_precision": ([i], i),
+        "accelerInt_escalated": ([], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -136,6 +138,11 @@ def set_precision(mode: int):
     """PRECISION_FP64 (default) or PRECISION_MIXED (FP32 iteration-matrix LU and
     Newton solves; FP64 residuals, error estimate, step control and state)."""
     _check(lib().accelerInt_set_precision(int(mode)), "accelerInt_set_precision")
+
+
+def escalated() -> int:
+    """Conditions of the last mixed-precision integration that were re-run with FP64 factors."""
+    return _check(lib().accelerInt_escalated(), "accelerInt_escalated")
 
 
 def set_abort_on_error(on: bool):

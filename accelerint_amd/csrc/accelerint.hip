@@ -56,12 +56,20 @@ __device__ __forceinline__ int grab_condition(int* work) {
   return Grp<G>::bcast_i(c, 0);
 }
 
+//
+// Escalation (mixed precision): with `esc` set, a condition that ends with a
+// non-zero code or a non-finite state is not written back; its index is
+// appended to esc[1..] (esc[0] = count) for an FP64 pass, which runs the same
+// kernel over that list (`list`, with its length read from `num_dev`) from the
+// untouched initial state.
 template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
     int num, double t_start, double t_end, double step, const double* __restrict__ pr,
     double* __restrict__ y, int* __restrict__ codes, uint32_t* __restrict__ counters,
-    int* __restrict__ work, SolverParams P, typename Mech::Params mp) {
+    int* __restrict__ work, const int* __restrict__ list, const int* __restrict__ num_dev,
+    int* __restrict__ esc, SolverParams P, typename Mech::Params mp) {
   extern __shared__ double lds_dyn[];
+  if (num_dev) num = *num_dev;
   constexpr int N = Mech::N;
   const int g = threadIdx.x / G;
   const int ln = threadIdx.x % G;
@@ -72,11 +80,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)
   uint32_t cnt[NCNT];
   double prc = 0.0, t_next = t_end;
   int k = 0;
-  int c = grab_condition<G>(work);
+  int ci = grab_condition<G>(work);
+  int c = 0;
   bool fresh = true;
 #pragma unroll 1
-  while (c < num) {
+  while (ci < num) {
     if (fresh) {  // load condition c and open its first global step
+      c = list ? list[ci] : ci;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int r = ln + G * q;
@@ -100,25 +110,40 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)
       st.begin(P, t0, t_next);
       continue;
     }
-    // condition finished (all steps done, or an error code): write back
+    // condition finished (all steps done, or an error code): write back, or
+    // hand it to the FP64 pass
+    bool escalate = false;
+    if (esc) {
+      double bad = 0.0;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int r = ln + G * q;
-      if (r < N) y[(size_t)c * N + r] = st.y[q];
+      for (int q = 0; q < R; ++q)
+        if (ln + G * q < N && !isfinite(st.y[q])) bad = 1.0;
+      escalate = code != EC_success || Grp<G>::sum(bad) != 0.0;
+    }
+    if (!escalate) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = ln + G * q;
+        if (r < N) y[(size_t)c * N + r] = st.y[q];
+      }
     }
     if (ln == 0) {
-      if (codes) codes[c] = code;
+      if (escalate) esc[1 + atomicAdd(esc, 1)] = c;
+      else if (codes) codes[c] = code;
       if (counters) {
 #pragma unroll
         for (int i = 0; i < NCNT; ++i) counters[(size_t)c * NCNT + i] += cnt[i];
       }
     }
-    c = grab_condition<G>(work);
+    ci = grab_condition<G>(work);
     fresh = true;
   }
 }
 
-__global__ void k_reset_work(int* work) { *work = 0; }
+__global__ void k_reset_work(int* work, int* esc) {
+  *work = 0;
+  if (esc) *esc = 0;
+}
 
 // Evaluates the mechanism callbacks at every condition's state (no
 // integration): dy[c*N + i] and jac[c*N*N + i + j*N] (column-major per
@@ -215,7 +240,8 @@ std::string g_launch_msg;
 // per-block limit and leave at least one resident block per CU.
 template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
 hipError_t launch_radau(int num, double t_start, double t_end, double step, const double* pr,
-                        double* y, int* codes, uint32_t* cnt, int* work, const SolverParams& P,
+                        double* y, int* codes, uint32_t* cnt, int* work, const int* list,
+                        const int* num_dev, int* esc, const SolverParams& P,
                         const typename Mech::Params& mp, hipStream_t s) {
   constexpr int GPB = BLOCK / G;
   auto kern = k_radau<Mech, G, R, BLOCK, WPE, MT>;
@@ -249,9 +275,9 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
   }
   int blocks = (num + GPB - 1) / GPB;
   if (blocks > resident) blocks = resident;
-  hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work);
+  hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work, esc);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, num, t_start, t_end, step, pr, y, codes,
-                     cnt, work, P, mp);
+                     cnt, work, list, num_dev, esc, P, mp);
   return hipGetLastError();
 }
 
@@ -277,6 +303,7 @@ struct State {
   int* d_codes = nullptr;
   uint32_t* d_cnt = nullptr;
   int* d_work = nullptr;    // atomic work counter of the persistent kernel
+  int* d_esc = nullptr;     // escalation list of the mixed-precision pass: [count, indices...]
   std::vector<int> h_codes;
   int have_status = 0;
   // kinetics tables (device copies)
@@ -309,8 +336,8 @@ bool hip_ok(hipError_t e, const char* what) {
 #define HIPOK(x) hip_ok((x), #x)
 
 size_t bytes_per_condition() {
-  // y (condition-major) + staging copy + parameter + code + counters
-  return (size_t)S.n * sizeof(double) * 2 + sizeof(double) + sizeof(int) + NCNT * sizeof(uint32_t);
+  // y (condition-major) + staging copy + parameter + code + escalation slot + counters
+  return (size_t)S.n * sizeof(double) * 2 + sizeof(double) + 2 * sizeof(int) + NCNT * sizeof(uint32_t);
 }
 
 void free_buffers() {
@@ -320,7 +347,9 @@ void free_buffers() {
   if (S.d_codes) hipFree(S.d_codes);
   if (S.d_cnt) hipFree(S.d_cnt);
   if (S.d_work) hipFree(S.d_work);
+  if (S.d_esc) hipFree(S.d_esc);
   S.d_work = nullptr;
+  S.d_esc = nullptr;
   S.d_y = S.d_ycm = S.d_pr = nullptr;
   S.d_codes = nullptr;
   S.d_cnt = nullptr;
@@ -336,6 +365,7 @@ bool alloc_buffers(int cap) {
   if (!HIPOK(hipMalloc(&S.d_cnt, (size_t)cap * NCNT * sizeof(uint32_t)))) return false;
   if (!HIPOK(hipMemset(S.d_cnt, 0, (size_t)cap * NCNT * sizeof(uint32_t)))) return false;
   if (!HIPOK(hipMalloc(&S.d_work, sizeof(int)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_esc, ((size_t)cap + 1) * sizeof(int)))) return false;
   S.padded = cap;
   S.h_codes.assign(cap, 0);
   return true;
@@ -346,30 +376,37 @@ bool alloc_buffers(int cap) {
 // a whole wave per condition, four waves per block sharing the LDS tables.
 template <class MT>
 hipError_t launch_mt(int num, double t_start, double t_end, double step, const double* pr, double* y,
-                     int* codes, uint32_t* cnt, hipStream_t s) {
+                     int* codes, uint32_t* cnt, const int* list, const int* num_dev, int* esc,
+                     hipStream_t s) {
   switch (S.mech) {
     case ACCELERINT_MECH_VAN_DER_POL:
       return launch_radau<MechVdP, 1, 2, 256, ACC_VDP_WPE, MT>(num, t_start, t_end, step, pr, y, codes, cnt,
-                                                               S.d_work, S.P, MechVdP::Params{}, s);
+                                                               S.d_work, list, num_dev, esc, S.P,
+                                                               MechVdP::Params{}, s);
     case ACCELERINT_MECH_KINETICS:
       if (S.n == 14)
-        return launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE, MT>(num, t_start, t_end, step, pr, y,
-                                                                         codes, cnt, S.d_work, S.P, S.kp14, s);
+        return launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE, MT>(
+            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp14, s);
       if (S.n == 54)
-        return launch_radau<MechKinetics54, 64, 1, 256, ACC_K54_WPE, MT>(num, t_start, t_end, step, pr, y,
-                                                                         codes, cnt, S.d_work, S.P, S.kp54, s);
+        return launch_radau<MechKinetics54, 64, 1, 256, ACC_K54_WPE, MT>(
+            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp54, s);
       return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
   }
 }
 
+// One integration of num <= S.padded conditions.  Mixed precision runs the
+// FP32-factor kernel over all conditions, then the FP64 kernel over the
+// conditions it escalated (the count stays on the device: no host sync).
 hipError_t launch(int num, double t_start, double t_end, double step, const double* pr, double* y,
                   int* codes, uint32_t* cnt, hipStream_t s) {
   g_launch_msg.clear();
-  if (S.precision == ACCELERINT_PRECISION_MIXED)
-    return launch_mt<float>(num, t_start, t_end, step, pr, y, codes, cnt, s);
-  return launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, s);
+  if (S.precision != ACCELERINT_PRECISION_MIXED)
+    return launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, nullptr, s);
+  hipError_t e = launch_mt<float>(num, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, S.d_esc, s);
+  if (e != hipSuccess) return e;
+  return launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, S.d_esc + 1, S.d_esc, nullptr, s);
 }
 
 hipError_t launch_eval_any(int num, double t, const double* pr, const double* y, double* dy,
@@ -392,7 +429,13 @@ int run_steps(int num, double t_start, double t_end, double stepsize, double* y,
               int* codes, uint32_t* cnt, hipStream_t s) {
   const double step = stepsize < 0 ? t_end - t_start : stepsize;
   if (!(t_start + kRoundoff < t_end) || num <= 0) return 0;
-  hipError_t e = launch(num, t_start, t_end, step, pr, y, codes, cnt, s);
+  hipError_t e = hipSuccess;
+  // chunks of at most `padded` conditions (the escalation list's capacity)
+  for (int off = 0; off < num && e == hipSuccess; off += S.padded) {
+    const int m = num - off < S.padded ? num - off : S.padded;
+    e = launch(m, t_start, t_end, step, pr + off, y + (size_t)off * S.n, codes ? codes + off : nullptr,
+               cnt ? cnt + (size_t)off * NCNT : nullptr, s);
+  }
   if (e != hipSuccess && !g_launch_msg.empty()) {
     set_err("%s", g_launch_msg.c_str());
     if (S.abort_on_error) {
@@ -637,6 +680,14 @@ int accelerInt_eval_callbacks(int NUM, double t, const double* y_host, const dou
   hipFree(d_dy);
   hipFree(d_jac);
   return rc;
+}
+
+int accelerInt_escalated(void) {
+  if (S.device < 0 || !S.d_esc) return -1;
+  int n = 0;
+  if (!HIPOK(hipStreamSynchronize(S.stream))) return -1;
+  if (!HIPOK(hipMemcpy(&n, S.d_esc, sizeof(int), hipMemcpyDeviceToHost))) return -1;
+  return n;
 }
 
 int accelerInt_synchronize(void) {

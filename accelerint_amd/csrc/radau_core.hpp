@@ -208,8 +208,6 @@ __device__ __forceinline__ void crecip(T c, T d, T& re, T& im) {
   }
 }
 
-// Argmax over the group of (value, key) with key = (LAPACK position << 8) | row;
-// larger value wins, equal values go to the smaller position (first maximum).
 // DPP lane exchanges inside a 16-lane row: quad_perm [1,0,3,2] (xor 1),
 // quad_perm [2,3,0,1] (xor 2), row_half_mirror (i <-> 7 - i in each 8) and
 // row_mirror (i <-> 15 - i).  Applied in that order, a commutative pairwise
@@ -240,26 +238,25 @@ __device__ __forceinline__ void argmax_merge(T& bv, int& bkey, T ov, int ok) {
 // Argmax over the group of (value, key) with key = (LAPACK position << 8) | row;
 // larger value wins, equal values go to the smaller position (first maximum).
 // The order is total on non-NaN values, so every pairing network returns the
-// same winner: 16- and 64-lane groups use DPP within rows (plus, for 64, the
-// four row winners read through SGPRs); other widths an xor butterfly.
+// same winner: 16- and 64-lane groups use DPP within rows, other widths an
+// xor butterfly.
 template <int G, class T>
 __device__ __forceinline__ void group_argmax(T& bv, int& bkey) {
-  if constexpr (G == 16 || G == 64) {
+  if constexpr (G == 16) {
     argmax_merge(bv, bkey, dpp_t<kDppXor1>(bv), dpp_i<kDppXor1>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppXor2>(bv), dpp_i<kDppXor2>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
-    if constexpr (G == 64) {
-      T v0 = Grp<64>::bcast(bv, 0), v1 = Grp<64>::bcast(bv, 16);
-      T v2 = Grp<64>::bcast(bv, 32), v3 = Grp<64>::bcast(bv, 48);
-      int k0 = Grp<64>::bcast_i(bkey, 0), k1 = Grp<64>::bcast_i(bkey, 16);
-      int k2 = Grp<64>::bcast_i(bkey, 32), k3 = Grp<64>::bcast_i(bkey, 48);
-      argmax_merge(v0, k0, v1, k1);
-      argmax_merge(v2, k2, v3, k3);
-      argmax_merge(v0, k0, v2, k2);
-      bv = v0;
-      bkey = k0;
-    }
+  } else if constexpr (G == 64) {
+    // DPP within each 16-lane row, then the row winners by two xor exchanges
+    // (kept in VGPRs: merging readlane results in scalar registers gave wrong
+    // pivots on gfx950)
+    argmax_merge(bv, bkey, dpp_t<kDppXor1>(bv), dpp_i<kDppXor1>(bkey));
+    argmax_merge(bv, bkey, dpp_t<kDppXor2>(bv), dpp_i<kDppXor2>(bkey));
+    argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
+    argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
+    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 16), Grp<G>::xor_i(bkey, 16));
+    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 32), Grp<G>::xor_i(bkey, 32));
   } else {
 #pragma unroll
     for (int m = 1; m < G; m <<= 1) argmax_merge(bv, bkey, Grp<G>::xor_(bv, m), Grp<G>::xor_i(bkey, m));
@@ -860,6 +857,11 @@ struct Radau {
   // factorizations
   MT E1[R][N], E2r[R][N], E2i[R][N];
   int pos1[R], pos2[R];
+  // mixed precision: each row of E1 / E2 is scaled by 2^-e (exactly) before
+  // rounding to FP32 so its largest entry lies in [0.5, 1): Jacobian entries of
+  // stiff kinetics reach far beyond the FP32 range (reverse rate constants at
+  // trial temperatures), and row scaling changes no solution
+  int ex1[R], ex2[R];
   MT rd1[R], rd2r[R], rd2i[R];
   Perm<N> perm1, perm2;
   int src1, src2;  // column-LU path: original equation of this lane's row
@@ -873,7 +875,7 @@ struct Radau {
     } else {
       MT bb[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) bb[q] = (MT)b[q];
+      for (int q = 0; q < R; ++q) bb[q] = (MT)ldexp(b[q], -ex1[q]);
       solve_real<N, G, R, MT>(E1, pos1, perm1, rd1, bb);
 #pragma unroll
       for (int q = 0; q < R; ++q) b[q] = (double)bb[q];
@@ -887,7 +889,10 @@ struct Radau {
     } else {
       MT xr[R], xi[R];
 #pragma unroll
-      for (int q = 0; q < R; ++q) { xr[q] = (MT)br[q]; xi[q] = (MT)bi[q]; }
+      for (int q = 0; q < R; ++q) {
+        xr[q] = (MT)ldexp(br[q], -ex2[q]);
+        xi[q] = (MT)ldexp(bi[q], -ex2[q]);
+      }
       solve_cplx<N, G, R, MT>(E2r, E2i, pos2, perm2, rd2r, rd2i, xr, xi);
 #pragma unroll
       for (int q = 0; q < R; ++q) { br[q] = (double)xr[q]; bi[q] = (double)xi[q]; }
@@ -960,18 +965,38 @@ struct Radau {
       src2 = m2.get(ln);
     } else {
       if constexpr (Mech::kJacInLds) group_lds_sync();
+      auto jac_at = [&](int q, int r, int j) -> double {
+        if constexpr (Mech::kJacInLds) return RW::valid(ln, q) ? ctx.jac[r * N + j] : 0.0;
+        else return A[q][j];
+      };
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int r = RW::row(ln, q);
+        double s1 = 1.0, s2 = 1.0;
+        if constexpr (kMixed) {
+          double m1 = 0.0, m2 = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            const bool d = (j == r);
+            const double aj = jac_at(q, r, j);
+            m1 = fmax(m1, fabs(-aj + (d ? g1 : 0.0)));
+            m2 = fmax(m2, fabs(-aj + (d ? g2r : 0.0)) + (d ? g2i : 0.0));
+          }
+          int e1 = 0, e2 = 0;
+          if (m1 > 0.0) frexp(m1, &e1);
+          if (m2 > 0.0) frexp(m2, &e2);
+          ex1[q] = e1;
+          ex2[q] = e2;
+          s1 = ldexp(1.0, -e1);
+          s2 = ldexp(1.0, -e2);
+        }
 #pragma unroll
         for (int j = 0; j < N; ++j) {
           const bool d = (j == r);
-          double aj;
-          if constexpr (Mech::kJacInLds) aj = RW::valid(ln, q) ? ctx.jac[r * N + j] : 0.0;
-          else aj = A[q][j];
-          E1[q][j] = (MT)(-aj + (d ? g1 : 0.0));
-          E2r[q][j] = (MT)(-aj + (d ? g2r : 0.0));
-          E2i[q][j] = (MT)(d ? g2i : 0.0);
+          const double aj = jac_at(q, r, j);
+          E1[q][j] = (MT)((-aj + (d ? g1 : 0.0)) * s1);
+          E2r[q][j] = (MT)((-aj + (d ? g2r : 0.0)) * s2);
+          E2i[q][j] = (MT)((d ? g2i : 0.0) * s2);
         }
       }
       lu_real<N, G, R, MT>(E1, pos1, perm1, rd1, info);

@@ -187,7 +187,10 @@ def main():
             "algorithmic FLOPs per condition-step %.3g" % (
                 precision, ms, args.steps, wall, int((codes != 0).sum()), bool(np.isfinite(yf).all()),
                 flops / (num * args.steps)))
-        return {"wall": wall, "ms": ms, "codes": codes, "y": yf, "flops": flops}
+        esc = acc.escalated() if precision == "mixed" else 0
+        if precision == "mixed":
+            log("[mixed] conditions re-run with FP64 factors: %d" % esc)
+        return {"wall": wall, "ms": ms, "codes": codes, "y": yf, "flops": flops, "escalated": esc}
 
     precision = args.precision or DEFAULT_PRECISION[args.config]
     log("config %s: %s" % (args.config, desc))
@@ -199,7 +202,9 @@ def main():
         compare = {"plain_fp64_value": num * args.steps / ref["wall"],
                    "speedup_vs_plain_fp64": ref["wall"] / main_leg["wall"],
                    "max_weighted_diff_vs_fp64": float(diff.max()),
-                   "plain_fp64_nonzero_codes": int((ref["codes"] != 0).sum())}
+                   "plain_fp64_nonzero_codes": int((ref["codes"] != 0).sum()),
+                   "mixed_nonzero_codes": int((main_leg["codes"] != 0).sum()),
+                   "escalated_to_fp64": int(main_leg["escalated"])}
         log("compare: %s" % json.dumps(compare))
 
     if args.calib_copy_mb:

@@ -131,6 +131,13 @@ int accelerInt_integrate_device(int NUM, double t_start, double t_end, double st
 int accelerInt_eval_callbacks(int NUM, double t, const double *y_host, const double *var_host,
                               double *dy_host, double *jac_host);
 
+/* Mixed precision: number of conditions of the last integration (its last
+ * batch of at most NUM-at-initialize conditions) whose FP32-factor run ended
+ * with a non-zero code or a non-finite state and were therefore integrated
+ * again from their initial state with FP64 factors.  Waits for the library
+ * stream.  Returns -1 before initialize. */
+int accelerInt_escalated(void);
+
 /* Blocks until the library stream is idle; returns 0 or a negative error. */
 int accelerInt_synchronize(void);
 

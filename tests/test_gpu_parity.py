@@ -149,6 +149,8 @@ def _kin_parity(oracle_lib, mech, num, steps, dt, seed, T_range, precision, labe
         acc.integrate(num, 0.0, steps * dt, dt, y, P)
         codes = acc.get_status(num)
         cnt_g = acc.get_counters(num).astype(np.int64)
+        if precision == acc.PRECISION_MIXED:
+            print("%s: %d of %d conditions re-run with FP64 factors" % (label, acc.escalated(), num))
     finally:
         acc.set_precision(acc.PRECISION_FP64)
     om = oracle_lib.Mechanism.kinetics(mech)
