@@ -1,18 +1,24 @@
 """Benchmark: conditions/s per 1e-6 s global step of the batched Radau IIA-5
 integrator on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
 
-Workload (default ``c1``): the 14-state synthetic constant-pressure kinetics
-network (13 species + T, 27 reversible reactions), 262144 conditions with
-T ~ U[1000, 1600] K, P ~ U[1, 25] atm, Y ~ Dirichlet(0.5), plain FP64 Radau,
-rtol 1e-6, atol 1e-10, global steps of 1e-6 s.  A "step" is one global step of
-all conditions (one kernel launch); inputs are resident in HBM before timing.
+Workload (default ``c2``): the 54-state synthetic constant-pressure kinetics
+network (53 species + T, 325 reversible reactions), 131072 conditions with
+T ~ U[800, 2200] K, P ~ U[1, 25] atm, Y ~ Dirichlet(0.5), rtol 1e-6,
+atol 1e-10, global steps of 1e-6 s, mixed precision (FP32 iteration-matrix LU
+and Newton solves, FP64 residuals, error estimate, step control and state),
+with the plain FP64 method timed on the same inputs for comparison.  ``c1`` is
+the 14-state network (262144 conditions, plain FP64), ``c0`` van der Pol.
 
-Prints one JSON line.  ``roofline`` reports the integration kernel's
-algorithmic FP64 FLOP rate (from per-condition event counters) against the
-FP64 peak; ``cpu_baseline`` times the CPU oracle (C restatement of the
-reference's OpenMP radau2a.c path) on a bounded sample of the same workload.
+A step is one global step of every condition.  W warmup steps run untimed;
+the K timed steps then run as one persistent launch (plus, for mixed
+precision, the FP64 pass over escalated conditions) with inputs resident in
+HBM.  Prints one JSON line.  ``roofline`` reports the integration kernels'
+algorithmic FLOP rate (from per-condition event counters) against the FP64
+vector peak; ``cpu_baseline`` times the CPU oracle (C restatement of the
+reference's OpenMP radau2a.c path) on a bounded sample of the same timed
+workload.
 """
 from __future__ import annotations
 
@@ -28,6 +34,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
+PROFILES = os.path.join(ROOT, "profiles")
+# measured HBM traffic per condition-step of the integration kernel, by workload
+# and precision (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated): the profile
+# summaries under profiles/ carry the numbers and the commands that made them
+TRAFFIC_PROFILES = {("c2", "mixed"): "k_radau_kinetics54_mixed_131072.md"}
+
+
+def measured_traffic(config, precision):
+    """Bytes per condition-step recorded in the profile summary, or None."""
+    name = TRAFFIC_PROFILES.get((config, precision))
+    if not name or not os.path.exists(os.path.join(PROFILES, name)):
+        return None
+    for line in open(os.path.join(PROFILES, name)):
+        if "traffic_bytes_per_condition_step:" in line:
+            return float(line.split(":")[1].strip().strip("`"))
+    return None
 CONFIGS = {
     # name: (n_states, num, dt, T_range, description)
     "c0": (2, 4096, 1e-3, None, "van der Pol, mu=10^U(1,3), 4096 conditions, dt=1e-3"),
@@ -71,9 +93,12 @@ def algorithmic_flops(cnt, n, F, Jc):
             + es * (2 * n * n + 8 * n) + f * F + jac * Jc + step * 40 * n)
 
 
-def cpu_baseline(cfg, mech, y0, pr, dt, target_cpu_s=12.0):
-    """Times the CPU oracle with all host cores on a bounded sample of the
-    workload (first conditions, first global steps)."""
+def cpu_baseline(cfg, mech, y_start, pr, dt, steps, target_cpu_s=160.0):
+    """Times the CPU oracle (all host cores, at most 16) on a bounded sample of
+    the timed workload: the first conditions, starting from the same
+    post-warmup states the GPU's timed launch starts from, over the same
+    number of global steps (the right-hand sides are autonomous, so the
+    interval's origin does not matter)."""
     import oracle
     oracle.build()
     n = cfg[0]
@@ -84,28 +109,30 @@ def cpu_baseline(cfg, mech, y0, pr, dt, target_cpu_s=12.0):
     except AttributeError:
         pass
     cores = min(cores, 16)
-    # probe
-    num = min(1024, y0.shape[1])
+    y_start = np.ascontiguousarray(y_start)
+    # probe one global step on a small slice, then size the sample
+    num = min(cores * 32, y_start.shape[1])
     t0 = time.perf_counter()
-    oracle.integrate_steps(om, y0[:, :num], pr[:num], dt, 2, nthreads=cores)
-    rate = num * 2 / max(time.perf_counter() - t0, 1e-6)
-    steps = 20 if n != 2 else 100
-    num = int(min(y0.shape[1], max(cores * 64, rate * target_cpu_s / cores / steps)))
+    oracle.integrate_steps(om, np.ascontiguousarray(y_start[:, :num]), pr[:num], dt, 1, nthreads=cores)
+    per_cond_step = (time.perf_counter() - t0) * cores / num  # CPU-seconds per condition-step
+    num = int(min(y_start.shape[1], max(cores * 4, target_cpu_s / max(per_cond_step * steps, 1e-9))))
     num = max(cores, num - num % cores)
     t0 = time.perf_counter()
-    oracle.integrate_steps(om, y0[:, :num], pr[:num], dt, steps, nthreads=cores)
+    _, codes, _ = oracle.integrate_steps(om, np.ascontiguousarray(y_start[:, :num]), pr[:num], dt, steps,
+                                         nthreads=cores)
     wall = time.perf_counter() - t0
     return {"value": num * steps / wall, "unit": "conditions*steps/s", "cores": cores, "kind": "port",
-            "sample": "CPU oracle (C/OpenMP restatement of radau2a.c), first %d conditions x %d global "
-                      "steps of dt=%g, %d OpenMP threads, %.2f s wall" % (num, steps, dt, cores, wall)}
+            "sample": "CPU oracle (C/OpenMP restatement of radau2a.c, plain FP64), first %d conditions "
+                      "from the post-warmup states x %d global steps of dt=%g, %d OpenMP threads, %.2f s wall, "
+                      "%d nonzero codes" % (num, steps, dt, cores, wall, int((codes != 0).sum()))}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c1", choices=sorted(CONFIGS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--num", type=int, default=0, help="override condition count")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
                     help="iteration-matrix arithmetic (default: mixed for c2, fp64 otherwise)")
@@ -166,6 +193,7 @@ def main():
             t += args.warmup * dt
         torch.cuda.synchronize()
         acc.reset_counters()
+        y_warm = y_dev.cpu().numpy().T.copy()
         torch.cuda.synchronize()
         start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
@@ -190,7 +218,8 @@ def main():
         esc = acc.escalated() if precision == "mixed" else 0
         if precision == "mixed":
             log("[mixed] conditions re-run with FP64 factors: %d" % esc)
-        return {"wall": wall, "ms": ms, "codes": codes, "y": yf, "flops": flops, "escalated": esc}
+        return {"wall": wall, "ms": ms, "codes": codes, "y": yf, "flops": flops, "escalated": esc,
+                "y_warm": y_warm}
 
     precision = args.precision or DEFAULT_PRECISION[args.config]
     log("config %s: %s" % (args.config, desc))
@@ -217,9 +246,10 @@ def main():
         log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mb, args.calib_copy_mb))
 
     achieved_tflops = main_leg["flops"] / (main_leg["ms"] * 1e-3) / 1e12
+    traffic = measured_traffic(args.config, precision)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, mech, y0, pr, dt)
+        cpu = cpu_baseline(cfg, mech, main_leg["y_warm"], pr, dt, args.steps)
         log("cpu baseline: %s" % json.dumps(cpu))
     wall = main_leg["wall"]
     method = ("mixed-precision Radau IIA-5 (FP32 iteration-matrix LU and Newton solves, FP64 residuals, "
@@ -245,7 +275,8 @@ def main():
         # algorithmic FLOPs (both precisions) over the FP64 vector peak: FP32
         # non-packed VALU issues at the same rate as FP64 on this chip
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+                     "traffic": (traffic * num * args.steps) if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
