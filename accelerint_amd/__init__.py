@@ -40,6 +40,10 @@ EXPORTS = (
     "accelerInt_num_states", "accelerInt_integrate_device", "accelerInt_synchronize",
     "accelerInt_last_error", "accelerInt_eval_callbacks", "accelerInt_set_precision",
     "accelerInt_escalated",
+    "accelerInt_mlp_init", "accelerInt_mlp_param_count", "accelerInt_mlp_get_params",
+    "accelerInt_mlp_set_params", "accelerInt_mlp_set_normalization", "accelerInt_mlp_loss_grad",
+    "accelerInt_mlp_predict", "accelerInt_mlp_train", "accelerInt_mlp_last_error", "accelerInt_mlp_release",
+    "accelerInt_set_learned_start", "accelerInt_log_first_steps", "accelerInt_mlp_predict_states",
 )
 
 
@@ -85,6 +89,19 @@ def lib():
         "accelerInt_eval_callbacks": ([i, d, v, v, v, v], i),
         "accelerInt_set_precision": ([i], i),
         "accelerInt_escalated": ([], i),
+        "accelerInt_mlp_init": ([i, ctypes.c_uint], i),
+        "accelerInt_mlp_param_count": ([], i),
+        "accelerInt_mlp_get_params": ([v, i], i),
+        "accelerInt_mlp_set_params": ([v, i], i),
+        "accelerInt_mlp_set_normalization": ([v, v, v], i),
+        "accelerInt_mlp_loss_grad": ([i, v, v, v, v], i),
+        "accelerInt_mlp_predict": ([i, v, v], i),
+        "accelerInt_mlp_train": ([i, v, v, i, i, ctypes.c_float, ctypes.c_uint, v], i),
+        "accelerInt_mlp_last_error": ([], ctypes.c_char_p),
+        "accelerInt_mlp_release": ([], None),
+        "accelerInt_set_learned_start": ([i], i),
+        "accelerInt_log_first_steps": ([v, v, i, v], i),
+        "accelerInt_mlp_predict_states": ([i, d, v, v], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -143,6 +160,83 @@ def set_precision(mode: int):
 def escalated() -> int:
     """Conditions of the last mixed-precision integration that were re-run with FP64 factors."""
     return _check(lib().accelerInt_escalated(), "accelerInt_escalated")
+
+
+def _mlp_check(rc: int, what: str):
+    if rc < 0:
+        raise AccelerIntError("%s failed: %s" % (what, lib().accelerInt_mlp_last_error().decode()))
+    return rc
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_init(n_states: int, seed: int = 0):
+    """Fresh learned-start network for an n_states system (Glorot weights)."""
+    _mlp_check(lib().accelerInt_mlp_init(int(n_states), int(seed)), "accelerInt_mlp_init")
+
+
+def mlp_params() -> np.ndarray:
+    n = _mlp_check(lib().accelerInt_mlp_param_count(), "accelerInt_mlp_param_count")
+    out = np.zeros(n, dtype=np.float32)
+    _mlp_check(lib().accelerInt_mlp_get_params(out.ctypes.data, n), "accelerInt_mlp_get_params")
+    return out
+
+
+def mlp_set_params(p):
+    p = _f32(p)
+    _mlp_check(lib().accelerInt_mlp_set_params(p.ctypes.data, p.size), "accelerInt_mlp_set_params")
+
+
+def mlp_set_normalization(mu, sd, zscale):
+    mu, sd, zscale = _f32(mu), _f32(sd), _f32(zscale)
+    _mlp_check(lib().accelerInt_mlp_set_normalization(mu.ctypes.data, sd.ctypes.data, zscale.ctypes.data),
+               "accelerInt_mlp_set_normalization")
+
+
+def mlp_loss_grad(x, t):
+    """(loss, grad) of a normalized batch x[B][N+1], t[B][3N] (B a multiple of 16)."""
+    x, t = _f32(x), _f32(t)
+    n = _mlp_check(lib().accelerInt_mlp_param_count(), "accelerInt_mlp_param_count")
+    g = np.zeros(n, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _mlp_check(lib().accelerInt_mlp_loss_grad(x.shape[0], x.ctypes.data, t.ctypes.data, g.ctypes.data,
+                                              loss.ctypes.data), "accelerInt_mlp_loss_grad")
+    return float(loss[0]), g
+
+
+def mlp_predict(x, n_out: int):
+    x = _f32(x)
+    y = np.zeros((x.shape[0], n_out), dtype=np.float32)
+    _mlp_check(lib().accelerInt_mlp_predict(x.shape[0], x.ctypes.data, y.ctypes.data), "accelerInt_mlp_predict")
+    return y
+
+
+def mlp_train(n: int, x_dev_ptr: int, t_dev_ptr: int, epochs: int, batch: int, lr: float, seed: int = 0):
+    """Adam on device-resident normalized samples; returns the per-epoch loss."""
+    hist = np.zeros(epochs, dtype=np.float32)
+    _mlp_check(lib().accelerInt_mlp_train(int(n), x_dev_ptr, t_dev_ptr, int(epochs), int(batch), float(lr),
+                                          int(seed), hist.ctypes.data), "accelerInt_mlp_train")
+    return hist
+
+
+def mlp_predict_states(y_cond_major, h: float):
+    """The integrator's in-kernel inference on states y[num][N] (54-state networks)."""
+    y = np.ascontiguousarray(y_cond_major, dtype=np.float64)
+    z = np.zeros((y.shape[0], 3 * y.shape[1]), dtype=np.float64)
+    _mlp_check(lib().accelerInt_mlp_predict_states(y.shape[0], float(h), y.ctypes.data, z.ctypes.data),
+               "accelerInt_mlp_predict_states")
+    return z
+
+
+def set_learned_start(on: bool):
+    _check(lib().accelerInt_set_learned_start(int(bool(on))), "accelerInt_set_learned_start")
+
+
+def log_first_steps(x_dev_ptr: int, z_dev_ptr: int, capacity: int, count_dev_ptr: int):
+    _check(lib().accelerInt_log_first_steps(x_dev_ptr, z_dev_ptr, int(capacity), count_dev_ptr),
+           "accelerInt_log_first_steps")
 
 
 def set_abort_on_error(on: bool):

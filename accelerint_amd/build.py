@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaccelerint_hip.so")
-SOURCES = [os.path.join(CSRC, "accelerint.hip")]
+SOURCES = [os.path.join(CSRC, "accelerint.hip"), os.path.join(CSRC, "mlp.hip")]
 HEADERS = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")] + [
     os.path.join(ROOT, "include", "accelerint.h")]
 

@@ -62,6 +62,19 @@ __device__ __forceinline__ int grab_condition(int* work) {
 // appended to esc[1..] (esc[0] = count) for an FP64 pass, which runs the same
 // kernel over that list (`list`, with its length read from `num_dev`) from the
 // untouched initial state.
+// Learned Newton start for the first step of a global step (one condition
+// per wavefront; the network's system size must match).
+template <class Mech, int G, int R, class St>
+__device__ __forceinline__ void predict_start(const SolverParams& P, St& st) {
+  if constexpr (G == 64 && R == 1 && Mech::N < 64) {
+    if (P.mlp.on && P.mlp.n == Mech::N) {
+      double z1[1], z2[1], z3[1];
+      mlp_predict_wave<Mech::N>(P.mlp, st.y[0], log(st.H), z1[0], z2[0], z3[0]);
+      st.set_start(z1, z2, z3);
+    }
+  }
+}
+
 template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
     int num, double t_start, double t_end, double step, const double* __restrict__ pr,
@@ -98,6 +111,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)
       k = 0;
       t_next = fmin(t_end, t_start + step);
       st.begin(P, t_start, t_next);
+      predict_start<Mech, G, R>(P, st);
       fresh = false;
     }
     int code = EC_success;
@@ -108,6 +122,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)
       ++k;
       t_next = fmin(t_end, t_start + (k + 1) * step);
       st.begin(P, t0, t_next);
+      predict_start<Mech, G, R>(P, st);
       continue;
     }
     // condition finished (all steps done, or an error code): write back, or
@@ -292,6 +307,7 @@ struct State {
   int mech = ACCELERINT_MECH_VAN_DER_POL;
   int n = 2;
   int precision = ACCELERINT_PRECISION_FP64;
+  int learned = 0;          // learned Newton start on (the network is looked up per launch)
   SolverParams P{1e-10, 1e-6, 0};
   int abort_on_error = 1;
   int device = -1;
@@ -402,6 +418,14 @@ hipError_t launch_mt(int num, double t_start, double t_end, double step, const d
 hipError_t launch(int num, double t_start, double t_end, double step, const double* pr, double* y,
                   int* codes, uint32_t* cnt, hipStream_t s) {
   g_launch_msg.clear();
+  S.P.mlp = MlpView{};
+  if (S.learned) {
+    S.P.mlp = mlp_view();
+    if (!S.P.mlp.on || S.P.mlp.n != S.n) {
+      g_launch_msg = "learned start is on but no network for this system size is loaded";
+      return hipErrorInvalidValue;
+    }
+  }
   if (S.precision != ACCELERINT_PRECISION_MIXED)
     return launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, nullptr, s);
   hipError_t e = launch_mt<float>(num, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, S.d_esc, s);
@@ -520,6 +544,31 @@ int accelerInt_set_precision(int mode) {
     return -1;
   }
   S.precision = mode;
+  return 0;
+}
+
+int accelerInt_set_learned_start(int on) {
+  if (on) {
+    const MlpView v = mlp_view();
+    if (!v.on || v.n != S.n) {
+      set_err("accelerInt_set_learned_start: no network for a %d-state system (accelerInt_mlp_init / _train first)", S.n);
+      return -1;
+    }
+    if (S.n >= 64 || (S.mech == ACCELERINT_MECH_KINETICS && S.n != 54)) {
+      set_err("accelerInt_set_learned_start: the learned start runs with one condition per wavefront (54-state kinetics)");
+      return -1;
+    }
+  }
+  S.learned = on ? 1 : 0;
+  return 0;
+}
+
+int accelerInt_log_first_steps(float* x_dev, float* z_dev, int capacity, int* count_dev) {
+  if (capacity > 0 && (!x_dev || !z_dev || !count_dev)) {
+    set_err("accelerInt_log_first_steps: null buffer");
+    return -1;
+  }
+  S.P.log = capacity > 0 ? StartLog{x_dev, z_dev, count_dev, capacity} : StartLog{nullptr, nullptr, nullptr, 0};
   return 0;
 }
 

@@ -1,0 +1,522 @@
+// Learned Newton start: BF16-MFMA training and inference kernels and the
+// network's host-side owner (see mlp.hpp).  No counterpart in the reference.
+//
+// Training: mean squared error over normalized stage targets, Adam, batches of
+// B samples (B a multiple of 16).  Every matrix product (forward and backward)
+// is one call of k_mlp_gemm: a 16x16 output tile per wavefront, operands read
+// as FP32 from memory with arbitrary strides (transposes are stride choices),
+// rounded to BF16 (nearest even) in registers, accumulated in FP32.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <random>
+#include <string>
+#include <vector>
+
+#include "accelerint.h"
+#include "mlp.hpp"
+
+namespace acc {
+
+// C[M x N] (+)= A'[M x K] * B'[K x N] with A'(m, k) = A[m*sam + k*sak] and
+// B'(k, n) = B[k*sbk + n*sbn]; M, N, K multiples of 16.  4 waves per block.
+__global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const float* __restrict__ A, int sam,
+                                                  int sak, const float* __restrict__ B, int sbk, int sbn,
+                                                  float* __restrict__ C, int ldc, int accumulate) {
+  const int wave = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  const int l = (int)(threadIdx.x & 63);
+  const int tiles_n = N / 16;
+  const int tm = wave / tiles_n, tn = wave % tiles_n;
+  if (tm >= M / 16) return;  // whole waves only
+  const int m = tm * 16 + (l & 15), n = tn * 16 + (l & 15), kq = 4 * (l >> 4);
+  mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    mlp_bf16x4 a, b;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = k0 + kq + i;
+      a[i] = (short)mlp_bf16(A[(size_t)m * sam + (size_t)k * sak]);
+      b[i] = (short)mlp_bf16(B[(size_t)k * sbk + (size_t)n * sbn]);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const size_t idx = (size_t)(tm * 16 + kq + i) * ldc + tn * 16 + (l & 15);
+    C[idx] = accumulate ? C[idx] + acc[i] : acc[i];
+  }
+}
+
+// Z[b][j] += bias[j] (j < valid; padding columns zeroed); A = gelu(Z) or Z.
+__global__ void k_mlp_bias_act(int rows, int cols, int valid, float* __restrict__ Z, const float* __restrict__ bias,
+                               float* __restrict__ Aout, int gelu) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)rows * cols) return;
+  const int j = (int)(idx % cols);
+  const float z = j < valid ? Z[idx] + bias[j] : 0.0f;
+  Z[idx] = z;
+  if (Aout) Aout[idx] = gelu ? (j < valid ? mlp_gelu(z) : 0.0f) : z;
+}
+
+// dY = 2 (Y - T) / (rows * valid) on valid columns; loss partial sums per block.
+__global__ void k_mlp_loss(int rows, int cols, int valid, const float* __restrict__ Y, const float* __restrict__ T,
+                           float* __restrict__ dY, double* __restrict__ loss) {
+  __shared__ double part[256];
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double e2 = 0.0;
+  if (idx < (size_t)rows * cols) {
+    const int b = (int)(idx / cols), j = (int)(idx % cols);
+    float d = 0.0f;
+    if (j < valid) {
+      const float e = Y[idx] - T[(size_t)b * valid + j];
+      d = 2.0f * e / (float)((double)rows * valid);
+      e2 = (double)e * e;
+    }
+    dY[idx] = d;
+  }
+  part[threadIdx.x] = e2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[blockIdx.x] = part[0];
+}
+
+// dZ = dA * gelu'(Z) (in place on dA).
+__global__ void k_mlp_gelu_back(size_t n, float* __restrict__ dA, const float* __restrict__ Z) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) dA[idx] *= mlp_gelu_grad(Z[idx]);
+}
+
+// out[j] = sum_b X[b][j] (deterministic: one thread per column).
+__global__ void k_mlp_colsum(int rows, int cols, const float* __restrict__ X, float* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  float s = 0.0f;
+  for (int b = 0; b < rows; ++b) s += X[(size_t)b * cols + j];
+  out[j] = s;
+}
+
+__global__ void k_mlp_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                           float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 K x N row-major -> BF16 in MFMA B-operand tile order (see MlpView).
+__global__ void k_mlp_swizzle(int K, int N, const float* __restrict__ W, uint16_t* __restrict__ out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= K * N) return;
+  const int KT = K / 16;
+  const int i = idx & 3, l = (idx >> 2) & 63, tile = idx >> 8;
+  const int kt = tile % KT, nt = tile / KT;
+  const int k = 16 * kt + 4 * (l >> 4) + i, n = 16 * nt + (l & 15);
+  out[idx] = mlp_bf16(W[(size_t)k * N + n]);
+}
+
+// Copies sample rows idx[b] of X (n x cols) into a padded batch (rows x ld).
+__global__ void k_mlp_gather(int rows, int cols, int ld, const int* __restrict__ idx, const float* __restrict__ X,
+                             float* __restrict__ out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)rows * ld) return;
+  const int b = (int)(t / ld), j = (int)(t % ld);
+  out[t] = j < cols ? X[(size_t)idx[b] * cols + j] : 0.0f;
+}
+
+// In-integrator inference path on given states: one wavefront per condition,
+// lane r < N holding y_r (condition-major y[c*N + r]); z[c*3N + s*N + r].
+template <int N>
+__global__ __launch_bounds__(64) void k_mlp_predict_states(int num, double logh, const double* __restrict__ y,
+                                                           double* __restrict__ z, MlpView m) {
+  const int c = blockIdx.x;
+  if (c >= num) return;
+  const int l = (int)threadIdx.x;
+  const double yr = l < N ? y[(size_t)c * N + l] : 0.0;
+  double z1, z2, z3;
+  mlp_predict_wave<N>(m, yr, logh, z1, z2, z3);
+  if (l < N) {
+    z[(size_t)c * 3 * N + l] = z1;
+    z[(size_t)c * 3 * N + N + l] = z2;
+    z[(size_t)c * 3 * N + 2 * N + l] = z3;
+  }
+}
+
+namespace {
+
+struct Mlp {
+  int n = 0;                 // system size
+  int in = 0, inp = 0, out = 0, outp = 0;
+  int off_w1 = 0, off_b1 = 0, off_w2 = 0, off_b2 = 0, off_w3 = 0, off_b3 = 0, count = 0;
+  float* p = nullptr;        // parameters (FP32 master)
+  float* g = nullptr;        // gradients
+  float* am = nullptr;       // Adam moments
+  float* av = nullptr;
+  uint16_t* w1s = nullptr;   // BF16 swizzled copies for the integrator
+  uint16_t* w2s = nullptr;
+  uint16_t* w3s = nullptr;
+  float* mu = nullptr;
+  float* sd = nullptr;
+  float* zs = nullptr;
+  int adam_t = 0;
+  // batch workspace
+  int cap = 0;
+  float *X = nullptr, *Z1 = nullptr, *A1 = nullptr, *Z2 = nullptr, *A2 = nullptr, *Y = nullptr, *T = nullptr,
+        *dY = nullptr, *dA2 = nullptr, *dA1 = nullptr;
+  double* loss = nullptr;
+  int* idx = nullptr;
+  std::string err;
+} G;
+
+bool ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  G.err = std::string(what) + ": " + hipGetErrorString(e);
+  return false;
+}
+
+void free_ws() {
+  for (float** q : {&G.X, &G.Z1, &G.A1, &G.Z2, &G.A2, &G.Y, &G.T, &G.dY, &G.dA2, &G.dA1}) {
+    if (*q) hipFree(*q);
+    *q = nullptr;
+  }
+  if (G.loss) hipFree(G.loss);
+  if (G.idx) hipFree(G.idx);
+  G.loss = nullptr;
+  G.idx = nullptr;
+  G.cap = 0;
+}
+
+void free_all() {
+  free_ws();
+  for (float** q : {&G.p, &G.g, &G.am, &G.av, &G.mu, &G.sd, &G.zs}) {
+    if (*q) hipFree(*q);
+    *q = nullptr;
+  }
+  for (uint16_t** q : {&G.w1s, &G.w2s, &G.w3s}) {
+    if (*q) hipFree(*q);
+    *q = nullptr;
+  }
+  G.n = 0;
+}
+
+bool ensure_ws(int B) {
+  if (B <= G.cap) return true;
+  free_ws();
+  const size_t H = kMlpHidden;
+  bool good = ok(hipMalloc(&G.X, (size_t)B * G.inp * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.Z1, (size_t)B * H * 4), "hipMalloc") && ok(hipMalloc(&G.A1, (size_t)B * H * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.Z2, (size_t)B * H * 4), "hipMalloc") && ok(hipMalloc(&G.A2, (size_t)B * H * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.Y, (size_t)B * G.outp * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.T, (size_t)B * G.out * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.dY, (size_t)B * G.outp * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.dA2, (size_t)B * H * 4), "hipMalloc") && ok(hipMalloc(&G.dA1, (size_t)B * H * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.loss, (size_t)((B * G.outp + 255) / 256) * sizeof(double)), "hipMalloc") &&
+              ok(hipMalloc(&G.idx, (size_t)B * sizeof(int)), "hipMalloc");
+  if (!good) {
+    free_ws();
+    return false;
+  }
+  G.cap = B;
+  return true;
+}
+
+int blocks_for(size_t n, int t = 256) { return (int)((n + t - 1) / t); }
+
+void gemm(int M, int N, int K, const float* A, int sam, int sak, const float* B, int sbk, int sbn, float* C,
+          int ldc, int accumulate) {
+  const int tiles = (M / 16) * (N / 16);
+  hipLaunchKernelGGL(k_mlp_gemm, dim3((tiles + 3) / 4), dim3(256), 0, 0, M, N, K, A, sam, sak, B, sbk, sbn, C,
+                     ldc, accumulate);
+}
+
+// Forward pass of a prepared batch in G.X (B x inp); outputs in G.Y (B x outp).
+void forward(int B) {
+  const int H = kMlpHidden;
+  float *W1 = G.p + G.off_w1, *b1 = G.p + G.off_b1, *W2 = G.p + G.off_w2, *b2 = G.p + G.off_b2,
+        *W3 = G.p + G.off_w3, *b3 = G.p + G.off_b3;
+  gemm(B, H, G.inp, G.X, G.inp, 1, W1, H, 1, G.Z1, H, 0);
+  hipLaunchKernelGGL(k_mlp_bias_act, dim3(blocks_for((size_t)B * H)), dim3(256), 0, 0, B, H, H, G.Z1, b1, G.A1, 1);
+  gemm(B, H, H, G.A1, H, 1, W2, H, 1, G.Z2, H, 0);
+  hipLaunchKernelGGL(k_mlp_bias_act, dim3(blocks_for((size_t)B * H)), dim3(256), 0, 0, B, H, H, G.Z2, b2, G.A2, 1);
+  gemm(B, G.outp, H, G.A2, H, 1, W3, G.outp, 1, G.Y, G.outp, 0);
+  hipLaunchKernelGGL(k_mlp_bias_act, dim3(blocks_for((size_t)B * G.outp)), dim3(256), 0, 0, B, G.outp, G.out, G.Y,
+                     b3, (float*)nullptr, 0);
+}
+
+// Loss and gradients (into G.g) of the batch in G.X / G.T; returns the summed
+// squared error.
+bool loss_grad(int B, double* sse) {
+  const int H = kMlpHidden;
+  forward(B);
+  const int nb = blocks_for((size_t)B * G.outp);
+  hipLaunchKernelGGL(k_mlp_loss, dim3(nb), dim3(256), 0, 0, B, G.outp, G.out, G.Y, G.T, G.dY, G.loss);
+  float *W2 = G.p + G.off_w2, *W3 = G.p + G.off_w3;
+  float *gW1 = G.g + G.off_w1, *gb1 = G.g + G.off_b1, *gW2 = G.g + G.off_w2, *gb2 = G.g + G.off_b2,
+        *gW3 = G.g + G.off_w3, *gb3 = G.g + G.off_b3;
+  // layer 3
+  gemm(H, G.outp, B, G.A2, 1, H, G.dY, G.outp, 1, gW3, G.outp, 0);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(blocks_for(G.outp)), dim3(256), 0, 0, B, G.outp, G.dY, gb3);
+  gemm(B, H, G.outp, G.dY, G.outp, 1, W3, 1, G.outp, G.dA2, H, 0);
+  hipLaunchKernelGGL(k_mlp_gelu_back, dim3(blocks_for((size_t)B * H)), dim3(256), 0, 0, (size_t)B * H, G.dA2, G.Z2);
+  // layer 2
+  gemm(H, H, B, G.A1, 1, H, G.dA2, H, 1, gW2, H, 0);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(blocks_for(H)), dim3(256), 0, 0, B, H, G.dA2, gb2);
+  gemm(B, H, H, G.dA2, H, 1, W2, 1, H, G.dA1, H, 0);
+  hipLaunchKernelGGL(k_mlp_gelu_back, dim3(blocks_for((size_t)B * H)), dim3(256), 0, 0, (size_t)B * H, G.dA1, G.Z1);
+  // layer 1
+  gemm(G.inp, H, B, G.X, 1, G.inp, G.dA1, H, 1, gW1, H, 0);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3(blocks_for(H)), dim3(256), 0, 0, B, H, G.dA1, gb1);
+  if (!ok(hipGetLastError(), "mlp backward")) return false;
+  std::vector<double> part(nb);
+  if (!ok(hipMemcpy(part.data(), G.loss, nb * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy")) return false;
+  double s = 0.0;
+  for (double v : part) s += v;
+  *sse = s;
+  return true;
+}
+
+bool refresh_swizzled() {
+  const int H = kMlpHidden;
+  hipLaunchKernelGGL(k_mlp_swizzle, dim3(blocks_for((size_t)G.inp * H)), dim3(256), 0, 0, G.inp, H, G.p + G.off_w1,
+                     G.w1s);
+  hipLaunchKernelGGL(k_mlp_swizzle, dim3(blocks_for((size_t)H * H)), dim3(256), 0, 0, H, H, G.p + G.off_w2, G.w2s);
+  hipLaunchKernelGGL(k_mlp_swizzle, dim3(blocks_for((size_t)H * G.outp)), dim3(256), 0, 0, H, G.outp,
+                     G.p + G.off_w3, G.w3s);
+  return ok(hipGetLastError(), "mlp swizzle") && ok(hipDeviceSynchronize(), "mlp swizzle");
+}
+
+// Uploads a host batch (x: B x in, t: B x out, normalized) into the padded workspace.
+bool upload_batch(int B, const float* x, const float* t) {
+  std::vector<float> xp((size_t)B * G.inp, 0.0f);
+  for (int b = 0; b < B; ++b)
+    for (int j = 0; j < G.in; ++j) xp[(size_t)b * G.inp + j] = x[(size_t)b * G.in + j];
+  if (!ok(hipMemcpy(G.X, xp.data(), xp.size() * 4, hipMemcpyHostToDevice), "hipMemcpy")) return false;
+  if (t && !ok(hipMemcpy(G.T, t, (size_t)B * G.out * 4, hipMemcpyHostToDevice), "hipMemcpy")) return false;
+  return true;
+}
+
+}  // namespace
+
+MlpView mlp_view() {
+  MlpView v{};
+  if (G.n == 0) return v;
+  v.w1 = G.w1s;
+  v.w2 = G.w2s;
+  v.w3 = G.w3s;
+  v.b1 = G.p + G.off_b1;
+  v.b2 = G.p + G.off_b2;
+  v.b3 = G.p + G.off_b3;
+  v.mu = G.mu;
+  v.sd = G.sd;
+  v.zs = G.zs;
+  v.n = G.n;
+  v.on = 1;
+  return v;
+}
+
+}  // namespace acc
+
+using namespace acc;
+
+extern "C" {
+
+int accelerInt_mlp_init(int n_states, unsigned seed) {
+  if (n_states < 1 || n_states + 1 > 64) {
+    G.err = "accelerInt_mlp_init: n_states must be in 1..63";
+    return -1;
+  }
+  free_all();
+  const int H = kMlpHidden;
+  G.n = n_states;
+  G.in = n_states + 1;
+  G.inp = mlp_pad16(G.in);
+  G.out = 3 * n_states;
+  G.outp = mlp_pad16(G.out);
+  G.off_w1 = 0;
+  G.off_b1 = G.off_w1 + G.inp * H;
+  G.off_w2 = G.off_b1 + H;
+  G.off_b2 = G.off_w2 + H * H;
+  G.off_w3 = G.off_b2 + H;
+  G.off_b3 = G.off_w3 + H * G.outp;
+  G.count = G.off_b3 + G.outp;
+  // Glorot-uniform weights over the valid rows/columns, zero padding and biases
+  std::vector<float> p(G.count, 0.0f);
+  std::mt19937 rng(seed);
+  auto fill = [&](int off, int K, int N, int kv, int nv) {
+    const float a = sqrtf(6.0f / (float)(kv + nv));
+    std::uniform_real_distribution<float> u(-a, a);
+    for (int k = 0; k < K; ++k)
+      for (int n = 0; n < N; ++n) p[off + k * N + n] = (k < kv && n < nv) ? u(rng) : 0.0f;
+  };
+  fill(G.off_w1, G.inp, H, G.in, H);
+  fill(G.off_w2, H, H, H, H);
+  fill(G.off_w3, H, G.outp, H, G.out);
+  std::vector<float> one(G.outp > G.inp ? G.outp : G.inp, 1.0f), zero(G.inp, 0.0f);
+  bool good = ok(hipMalloc(&G.p, G.count * 4), "hipMalloc") && ok(hipMalloc(&G.g, G.count * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.am, G.count * 4), "hipMalloc") && ok(hipMalloc(&G.av, G.count * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.w1s, (size_t)G.inp * H * 2), "hipMalloc") &&
+              ok(hipMalloc(&G.w2s, (size_t)H * H * 2), "hipMalloc") &&
+              ok(hipMalloc(&G.w3s, (size_t)H * G.outp * 2), "hipMalloc") &&
+              ok(hipMalloc(&G.mu, G.inp * 4), "hipMalloc") && ok(hipMalloc(&G.sd, G.inp * 4), "hipMalloc") &&
+              ok(hipMalloc(&G.zs, G.outp * 4), "hipMalloc") &&
+              ok(hipMemcpy(G.p, p.data(), G.count * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              ok(hipMemset(G.am, 0, G.count * 4), "hipMemset") && ok(hipMemset(G.av, 0, G.count * 4), "hipMemset") &&
+              ok(hipMemset(G.g, 0, G.count * 4), "hipMemset") &&
+              ok(hipMemcpy(G.mu, zero.data(), G.inp * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              ok(hipMemcpy(G.sd, one.data(), G.inp * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+              ok(hipMemcpy(G.zs, one.data(), G.outp * 4, hipMemcpyHostToDevice), "hipMemcpy") && refresh_swizzled();
+  if (!good) {
+    free_all();
+    return -1;
+  }
+  G.adam_t = 0;
+  return 0;
+}
+
+int accelerInt_mlp_param_count(void) { return G.n ? G.count : -1; }
+
+int accelerInt_mlp_get_params(float* out, int count) {
+  if (!G.n || count != G.count || !out) {
+    G.err = "accelerInt_mlp_get_params: no network or wrong count";
+    return -1;
+  }
+  return ok(hipMemcpy(out, G.p, (size_t)count * 4, hipMemcpyDeviceToHost), "hipMemcpy") ? 0 : -1;
+}
+
+int accelerInt_mlp_set_params(const float* in, int count) {
+  if (!G.n || count != G.count || !in) {
+    G.err = "accelerInt_mlp_set_params: no network or wrong count";
+    return -1;
+  }
+  if (!ok(hipMemcpy(G.p, in, (size_t)count * 4, hipMemcpyHostToDevice), "hipMemcpy")) return -1;
+  return refresh_swizzled() ? 0 : -1;
+}
+
+int accelerInt_mlp_set_normalization(const float* mu, const float* sd, const float* zscale) {
+  if (!G.n || !mu || !sd || !zscale) {
+    G.err = "accelerInt_mlp_set_normalization: no network or null array";
+    return -1;
+  }
+  std::vector<float> m(G.inp, 0.0f), s(G.inp, 1.0f), z(G.outp, 0.0f);
+  for (int j = 0; j < G.in; ++j) {
+    if (!(sd[j] > 0.0f)) {
+      G.err = "accelerInt_mlp_set_normalization: scales must be positive";
+      return -1;
+    }
+    m[j] = mu[j];
+    s[j] = sd[j];
+  }
+  for (int j = 0; j < G.out; ++j) z[j] = zscale[j];
+  return ok(hipMemcpy(G.mu, m.data(), G.inp * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+                 ok(hipMemcpy(G.sd, s.data(), G.inp * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+                 ok(hipMemcpy(G.zs, z.data(), G.outp * 4, hipMemcpyHostToDevice), "hipMemcpy")
+             ? 0
+             : -1;
+}
+
+int accelerInt_mlp_loss_grad(int B, const float* x, const float* t, float* grad, float* loss) {
+  if (!G.n || B <= 0 || B % 16 || !x || !t || !grad) {
+    G.err = "accelerInt_mlp_loss_grad: no network, or B not a positive multiple of 16";
+    return -1;
+  }
+  if (!ensure_ws(B) || !upload_batch(B, x, t)) return -1;
+  double sse = 0.0;
+  if (!loss_grad(B, &sse)) return -1;
+  if (loss) *loss = (float)(sse / ((double)B * G.out));
+  return ok(hipMemcpy(grad, G.g, (size_t)G.count * 4, hipMemcpyDeviceToHost), "hipMemcpy") ? 0 : -1;
+}
+
+int accelerInt_mlp_predict(int B, const float* x, float* y) {
+  if (!G.n || B <= 0 || B % 16 || !x || !y) {
+    G.err = "accelerInt_mlp_predict: no network, or B not a positive multiple of 16";
+    return -1;
+  }
+  if (!ensure_ws(B) || !upload_batch(B, x, nullptr)) return -1;
+  forward(B);
+  std::vector<float> yp((size_t)B * G.outp);
+  if (!ok(hipMemcpy(yp.data(), G.Y, yp.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy")) return -1;
+  for (int b = 0; b < B; ++b)
+    for (int j = 0; j < G.out; ++j) y[(size_t)b * G.out + j] = yp[(size_t)b * G.outp + j];
+  return 0;
+}
+
+int accelerInt_mlp_train(int n, const float* x_dev, const float* t_dev, int epochs, int batch, float lr,
+                         unsigned seed, float* loss_per_epoch) {
+  if (!G.n || n < batch || batch <= 0 || batch % 16 || epochs <= 0 || !x_dev || !t_dev) {
+    G.err = "accelerInt_mlp_train: bad arguments (batch a multiple of 16, n >= batch)";
+    return -1;
+  }
+  if (!ensure_ws(batch)) return -1;
+  std::mt19937 rng(seed);
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  int* d_perm = nullptr;
+  if (!ok(hipMalloc(&d_perm, (size_t)n * sizeof(int)), "hipMalloc")) return -1;
+  const int steps = n / batch;
+  int rc = 0;
+  for (int ep = 0; ep < epochs && rc == 0; ++ep) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    if (!ok(hipMemcpy(d_perm, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy")) {
+      rc = -1;
+      break;
+    }
+    double sse = 0.0;
+    for (int s = 0; s < steps; ++s) {
+      const int* ib = d_perm + (size_t)s * batch;
+      hipLaunchKernelGGL(k_mlp_gather, dim3(blocks_for((size_t)batch * G.inp)), dim3(256), 0, 0, batch, G.in,
+                         G.inp, ib, x_dev, G.X);
+      hipLaunchKernelGGL(k_mlp_gather, dim3(blocks_for((size_t)batch * G.out)), dim3(256), 0, 0, batch, G.out,
+                         G.out, ib, t_dev, G.T);
+      double part = 0.0;
+      if (!loss_grad(batch, &part)) {
+        rc = -1;
+        break;
+      }
+      sse += part;
+      ++G.adam_t;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.0f - powf(b1, (float)G.adam_t), c2 = 1.0f - powf(b2, (float)G.adam_t);
+      hipLaunchKernelGGL(k_mlp_adam, dim3(blocks_for(G.count)), dim3(256), 0, 0, G.count, G.p, G.g, G.am, G.av, lr,
+                         b1, b2, 1e-8f, c1, c2);
+    }
+    if (loss_per_epoch) loss_per_epoch[ep] = (float)(sse / ((double)steps * batch * G.out));
+  }
+  hipFree(d_perm);
+  if (rc == 0 && !refresh_swizzled()) rc = -1;
+  return rc;
+}
+
+int accelerInt_mlp_predict_states(int num, double h, const double* y, double* z) {
+  if (G.n != 54 || num <= 0 || !y || !z || !(h > 0)) {
+    G.err = "accelerInt_mlp_predict_states: needs a 54-state network, num > 0, h > 0";
+    return -1;
+  }
+  const int N = 54;
+  double *dy = nullptr, *dz = nullptr;
+  int rc = -1;
+  if (ok(hipMalloc(&dy, (size_t)num * N * 8), "hipMalloc") && ok(hipMalloc(&dz, (size_t)num * 3 * N * 8), "hipMalloc") &&
+      ok(hipMemcpy(dy, y, (size_t)num * N * 8, hipMemcpyHostToDevice), "hipMemcpy")) {
+    hipLaunchKernelGGL(k_mlp_predict_states<54>, dim3(num), dim3(64), 0, 0, num, log(h), dy, dz, mlp_view());
+    if (ok(hipGetLastError(), "k_mlp_predict_states") &&
+        ok(hipMemcpy(z, dz, (size_t)num * 3 * N * 8, hipMemcpyDeviceToHost), "hipMemcpy"))
+      rc = 0;
+  }
+  if (dy) hipFree(dy);
+  if (dz) hipFree(dz);
+  return rc;
+}
+
+const char* accelerInt_mlp_last_error(void) { return G.err.c_str(); }
+
+void accelerInt_mlp_release(void) { free_all(); }
+
+}  // extern "C"

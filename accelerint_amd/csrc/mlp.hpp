@@ -1,0 +1,181 @@
+// Learned Newton start: a small MLP that predicts the Radau stage values Z1..Z3
+// of a global step's first step from (state, log h).  No counterpart in the
+// reference (radau2a.c:600-603 starts every Newton iteration of a first step
+// at zero); the integrator keeps the reference's method as the fallback.
+//
+// Network: in = N + 1 features, two hidden layers of 128 (GELU, tanh form),
+// out = 3N; padded to multiples of 16 (IN_P, 128, OUT_P).  Matrix products run
+// on the BF16 MFMA units (v_mfma_f32_16x16x16_bf16) with FP32 accumulation;
+// master weights, biases and normalization stay FP32.
+//
+// MFMA operand layout (16x16x16, 64 lanes, 4 values per lane):
+//   A (M x K): lane l holds A[l % 16][4 (l / 16) + i]
+//   B (K x N): lane l holds B[4 (l / 16) + i][l % 16]
+//   D (M x N): lane l holds D[4 (l / 16) + i][l % 16]
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace acc {
+
+typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMlpHidden = 128;
+
+__host__ __device__ constexpr int mlp_pad16(int n) { return (n + 15) / 16 * 16; }
+
+// Round to nearest even bfloat16 (NaN kept quiet).
+__host__ __device__ inline uint16_t mlp_bf16(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t u = __float_as_uint(x);
+#else
+  uint32_t u;
+  __builtin_memcpy(&u, &x, 4);
+#endif
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ inline float mlp_gelu(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  return 0.5f * x * (1.0f + tanhf(k0 * (x + k1 * x * x * x)));
+}
+__device__ inline float mlp_gelu_grad(float x) {
+  const float k0 = 0.7978845608028654f, k1 = 0.044715f;
+  const float t = tanhf(k0 * (x + k1 * x * x * x));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k0 * (1.0f + 3.0f * k1 * x * x);
+}
+
+// Device view of a trained network, passed by value to the integrator.
+// Weights are BF16 in MFMA B-operand order: for layer L with KT = K/16 and
+// NT = N/16 tiles, tile (kt, nt) lane l holds W[16 kt + 4 (l / 16) + i][16 nt + l % 16]
+// at ((nt * KT + kt) * 64 + l) * 4 + i.
+struct MlpView {
+  const uint16_t* w1;
+  const uint16_t* w2;
+  const uint16_t* w3;
+  const float* b1;
+  const float* b2;
+  const float* b3;
+  const float* mu;   // [IN_P] feature means (features: y_0..y_{N-1}, log h)
+  const float* sd;   // [IN_P] feature scales
+  const float* zs;   // [OUT_P] output scales: Z_j = zs[j] * out_j
+  int n;             // system size N
+  int on;
+};
+
+// Training-sample log written by the integrator: the first accepted step of
+// each global step, features (y at the step start, log h) and the converged
+// stages (Z1, Z2, Z3), as float.
+struct StartLog {
+  float* x;      // [cap][N + 1]
+  float* z;      // [cap][3 N]
+  int* count;
+  int cap;
+};
+
+}  // namespace acc
+
+namespace acc {
+
+// One condition per wavefront (64-lane group, N <= 63): lane r < N holds y_r;
+// returns in lane r the predicted stages Z1[r], Z2[r], Z3[r] for a step of
+// size h.  Row 0 of each MFMA tile carries the condition; the other 15 rows
+// are zero (the per-wave cost is ~200 MFMA instructions).
+template <int N>
+__device__ inline void mlp_predict_wave(const MlpView& m, double yr, double logh, double& z1, double& z2,
+                                        double& z3) {
+  constexpr int IN = N + 1, INP = mlp_pad16(IN), OUT = 3 * N, OUTP = mlp_pad16(OUT), H = kMlpHidden;
+  constexpr int KT1 = INP / 16, NT1 = H / 16, NT3 = OUTP / 16;
+  static_assert(IN <= 64, "one feature per lane");
+  const int l = (int)__lane_id();
+  const bool row0 = (l & 15) == 0;
+  const int kq = 4 * (l >> 4);
+  float x = 0.0f;
+  if (l < IN) {
+    const float f = (l < N) ? (float)yr : (float)logh;
+    x = (f - m.mu[l]) / m.sd[l];
+  }
+  // layer 1: features -> 128
+  mlp_f32x4 h[NT1];
+#pragma unroll
+  for (int nt = 0; nt < NT1; ++nt) h[nt] = mlp_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kt = 0; kt < KT1; ++kt) {
+    mlp_bf16x4 a;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float v = __shfl(x, 16 * kt + kq + i);
+      a[i] = row0 ? (short)mlp_bf16(v) : (short)0;
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT1; ++nt) {
+      const mlp_bf16x4 b = *reinterpret_cast<const mlp_bf16x4*>(m.w1 + ((size_t)(nt * KT1 + kt) * 64 + l) * 4);
+      h[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, h[nt], 0, 0, 0);
+    }
+  }
+  float act[NT1];
+#pragma unroll
+  for (int nt = 0; nt < NT1; ++nt) act[nt] = mlp_gelu(h[nt][0] + m.b1[16 * nt + (l & 15)]);
+  // layer 2: 128 -> 128
+#pragma unroll
+  for (int nt = 0; nt < NT1; ++nt) h[nt] = mlp_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kt = 0; kt < NT1; ++kt) {
+    mlp_bf16x4 a;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float v = __shfl(act[kt], kq + i);
+      a[i] = row0 ? (short)mlp_bf16(v) : (short)0;
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT1; ++nt) {
+      const mlp_bf16x4 b = *reinterpret_cast<const mlp_bf16x4*>(m.w2 + ((size_t)(nt * NT1 + kt) * 64 + l) * 4);
+      h[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, h[nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int nt = 0; nt < NT1; ++nt) act[nt] = mlp_gelu(h[nt][0] + m.b2[16 * nt + (l & 15)]);
+  // layer 3: 128 -> 3N
+  mlp_f32x4 o[NT3];
+#pragma unroll
+  for (int nt = 0; nt < NT3; ++nt) o[nt] = mlp_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int kt = 0; kt < NT1; ++kt) {
+    mlp_bf16x4 a;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float v = __shfl(act[kt], kq + i);
+      a[i] = row0 ? (short)mlp_bf16(v) : (short)0;
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT3; ++nt) {
+      const mlp_bf16x4 b = *reinterpret_cast<const mlp_bf16x4*>(m.w3 + ((size_t)(nt * NT1 + kt) * 64 + l) * 4);
+      o[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, o[nt], 0, 0, 0);
+    }
+  }
+  // output j = s*N + r sits in lane j % 16 of tile j / 16
+  double zz[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int j = s * N + (l < N ? l : 0);
+    float v = 0.0f;
+#pragma unroll
+    for (int nt = 0; nt < NT3; ++nt) {
+      const float t = __shfl(o[nt][0], j & 15);
+      v = ((j >> 4) == nt) ? t : v;
+    }
+    zz[s] = (l < N) ? (double)((v + m.b3[j]) * m.zs[j]) : 0.0;
+  }
+  z1 = zz[0];
+  z2 = zz[1];
+  z3 = zz[2];
+}
+
+// Host side: the current trained network (nullptr-valued view with on = 0 if none).
+MlpView mlp_view();
+
+}  // namespace acc

@@ -22,6 +22,8 @@
 
 #include <type_traits>
 
+#include "mlp.hpp"
+
 namespace acc {
 
 // ---- control constants: radau2a.c:37-66 ------------------------------------
@@ -101,6 +103,8 @@ struct SolverParams {
   double atol;
   double rtol;
   int const_time_step;
+  MlpView mlp;    // learned Newton start of each global step's first step (on = 0: zero start)
+  StartLog log;   // training-sample log (cap = 0: off)
 };
 
 // ---- lane-group primitives ---------------------------------------------------
@@ -194,17 +198,39 @@ __device__ __forceinline__ constexpr T min_normal() {
   else return kDblMin;
 }
 
+// Reciprocal of the pivot: IEEE division in FP64 (bit-compatible with the
+// oracle's LAPACK semantics); in FP32 (mixed precision, no bit parity) the
+// hardware reciprocal refined by one Newton step, within 1 ulp.
+template <class T>
+__device__ __forceinline__ T recip(T x) {
+  if constexpr (std::is_same<T, float>::value) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(r, fmaf(-x, r, 1.0f), r);
+  } else {
+    return 1.0 / x;
+  }
+}
+
 // Smith's complex reciprocal 1/(c + i d).
 template <class T>
 __device__ __forceinline__ void crecip(T c, T d, T& re, T& im) {
-  if (fabs(c) >= fabs(d)) {
-    T r = d / c, den = c + d * r;
-    re = T(1) / den;
-    im = -r / den;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32 factors: the same formula with refined hardware reciprocals
+    const bool cd = fabsf(c) >= fabsf(d);
+    const float a = cd ? c : d, b = cd ? d : c;
+    const float r = b * recip(a), den = a + b * r, iden = recip(den);
+    re = cd ? iden : r * iden;
+    im = cd ? -r * iden : -iden;
   } else {
-    T r = c / d, den = c * r + d;
-    re = r / den;
-    im = T(-1) / den;
+    if (fabs(c) >= fabs(d)) {
+      T r = d / c, den = c + d * r;
+      re = T(1) / den;
+      im = -r / den;
+    } else {
+      T r = c / d, den = c * r + d;
+      re = r / den;
+      im = T(-1) / den;
+    }
   }
 }
 
@@ -225,6 +251,22 @@ __device__ __forceinline__ double dpp_t(double v) {
   return __hiloint2double(dpp_i<CTRL>(__double2hiint(v)), dpp_i<CTRL>(__double2loint(v)));
 }
 constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+// row_bcast:15 / row_bcast:31 (lane 15 of each row into the next row; lane 31
+// into rows 2 and 3), used with a row mask that leaves the other rows unchanged.
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_rows_i(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_rows_t(float old, float v) {
+  return __int_as_float(dpp_rows_i<CTRL, ROWS>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_rows_t(double old, double v) {
+  return __hiloint2double(dpp_rows_i<CTRL, ROWS>(__double2hiint(old), __double2hiint(v)),
+                          dpp_rows_i<CTRL, ROWS>(__double2loint(old), __double2loint(v)));
+}
 
 // (value, key) order of the pivot search: larger value, then smaller key.
 // Bitwise, not short-circuit, so the exchange stays branch-free.
@@ -248,15 +290,20 @@ __device__ __forceinline__ void group_argmax(T& bv, int& bkey) {
     argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
   } else if constexpr (G == 64) {
-    // DPP within each 16-lane row, then the row winners by two xor exchanges
-    // (kept in VGPRs: merging readlane results in scalar registers gave wrong
-    // pivots on gfx950)
+    // DPP within each 16-lane row, then the four row winners by row broadcasts
+    // (kept in VGPRs; only the final winner is read out: merging the four row
+    // winners after reading them into scalar registers gave wrong pivots on
+    // gfx950)
     argmax_merge(bv, bkey, dpp_t<kDppXor1>(bv), dpp_i<kDppXor1>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppXor2>(bv), dpp_i<kDppXor2>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
-    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 16), Grp<G>::xor_i(bkey, 16));
-    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 32), Grp<G>::xor_i(bkey, 32));
+    // rows 1 and 3 take the winners of rows 0 and 2, then rows 2 and 3 the
+    // winner of rows 0-1; lane 63 ends with the wave's winner
+    argmax_merge(bv, bkey, dpp_rows_t<kDppRowBcast15, 0xA>(bv, bv), dpp_rows_i<kDppRowBcast15, 0xA>(bkey, bkey));
+    argmax_merge(bv, bkey, dpp_rows_t<kDppRowBcast31, 0xC>(bv, bv), dpp_rows_i<kDppRowBcast31, 0xC>(bkey, bkey));
+    bkey = __builtin_amdgcn_readlane(bkey, 63);
+    bv = Grp<G>::bcast(bv, 63);
   } else {
 #pragma unroll
     for (int m = 1; m < G; m <<= 1) argmax_merge(bv, bkey, Grp<G>::xor_(bv, m), Grp<G>::xor_i(bkey, m));
@@ -332,8 +379,10 @@ __device__ __forceinline__ void lu_real(T (&a)[R][N], int (&pos)[R], Perm<N>& pe
     }
     const bool nz = piv != T(0);
     if (!nz && info == 0) info = k + 1;
-    const bool tiny = fabs(piv) < min_normal<T>();
-    const T rp = T(1) / piv;
+    // LAPACK divides instead of scaling by the reciprocal when |pivot| < sfmin
+    // (FP64 path only; the FP32 factors need no such care)
+    const bool tiny = std::is_same<T, double>::value && fabs(piv) < min_normal<T>();
+    const T rp = recip(piv);
     T l[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -827,6 +876,10 @@ __device__ __forceinline__ void rsolve_cplx(const double (&ar)[N], const double 
 // interleave step attempts of many conditions and refill a lane group with a
 // new condition or a new global step as soon as its current one ends.
 enum : int { RADAU_CONTINUE = -1 };
+// Internal code of the FP32-factor method: the factorization is too
+// inaccurate for its refined solves to be trusted; the condition goes to the
+// FP64 pass (never returned to callers).
+enum : int { EC_needs_fp64 = 64 };
 
 template <class Mech, int G, int R, class MT = double>
 struct Radau {
@@ -862,10 +915,25 @@ struct Radau {
   // stiff kinetics reach far beyond the FP32 range (reverse rate constants at
   // trial temperatures), and row scaling changes no solution
   int ex1[R], ex2[R];
+  // mixed precision: refine every Newton solve (1) or not (0); set to 1 by
+  // each factorization, cleared once a refined solve shows the FP32 factors
+  // accurate to 1e-6; a correction above 10 % of the solution flags `bad`
+  int refine;
+  bool bad;
   MT rd1[R], rd2r[R], rd2i[R];
   Perm<N> perm1, perm2;
   int src1, src2;  // column-LU path: original equation of this lane's row
   double A[R][N];  // Jacobian (only live across attempts when !Mech::kJacInLds)
+  // predicted Newton start of the current global step's first step; dropped
+  // (zero start, the reference's rule) after any rejection of that step
+  double Zs1[R], Zs2[R], Zs3[R];
+  bool UseStart;
+
+  __device__ __forceinline__ void set_start(const double (&z1)[R], const double (&z2)[R], const double (&z3)[R]) {
+#pragma unroll
+    for (int q = 0; q < R; ++q) { Zs1[q] = z1[q]; Zs2[q] = z2[q]; Zs3[q] = z3[q]; }
+    UseStart = true;
+  }
 
   __device__ __forceinline__ void solve1(double (&b)[R]) const {
     if constexpr (kColLU) {
@@ -936,6 +1004,79 @@ struct Radau {
     }
   }
 
+  // Newton solves (RK_Solve's two linear systems).  FP64 factors: one solve
+  // each.  FP32 factors: each solve is followed by one step of iterative
+  // refinement against the FP64 matrices gamma/H - J and (alpha + i beta)/H - J
+  // (the three residuals share one pass over the Jacobian's rows), so the
+  // simplified Newton iteration sees FP64-accurate increments and behaves as
+  // with FP64 factors whenever cond * 2^-24 << 1.  Without it, an iterate of
+  // an ill-conditioned system can pass the convergence test (whose rate comes
+  // from the previous step) with a sizeable error.
+  __device__ __forceinline__ void solve_newton(Ctx& ctx, double (&b1)[R], double (&b2)[R], double (&b3)[R]) {
+    if constexpr (!kMixed) {
+      solve1(b1);
+      solve2(b2, b3);
+    } else {
+      const int ln = Grp<G>::lane();
+      double x1[R], x2[R], x3[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) { x1[q] = b1[q]; x2[q] = b2[q]; x3[q] = b3[q]; }
+      solve1(x1);
+      solve2(x2, x3);
+      if constexpr (Mech::kJacInLds) group_lds_sync();
+      const double g1 = rkGamma / H, ar = rkAlpha / H, ai = rkBeta / H;
+      double r1[R], r2[R], r3[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        r1[q] = b1[q] - g1 * x1[q];
+        r2[q] = b2[q] - (ar * x2[q] - ai * x3[q]);
+        r3[q] = b3[q] - (ar * x3[q] + ai * x2[q]);
+      }
+      static_for<0, N>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const double xj1 = bcast_k<G, j % G>(x1[j / G]);
+        const double xj2 = bcast_k<G, j % G>(x2[j / G]);
+        const double xj3 = bcast_k<G, j % G>(x3[j / G]);
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          double Jrj;
+          if constexpr (Mech::kJacInLds) Jrj = RW::valid(ln, q) ? ctx.jac[RW::row(ln, q) * N + j] : 0.0;
+          else Jrj = A[q][j];
+          r1[q] = fma(Jrj, xj1, r1[q]);
+          r2[q] = fma(Jrj, xj2, r2[q]);
+          r3[q] = fma(Jrj, xj3, r3[q]);
+        }
+      });
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const bool v = RW::valid(ln, q);
+        r1[q] = v ? r1[q] : 0.0;
+        r2[q] = v ? r2[q] : 0.0;
+        r3[q] = v ? r3[q] : 0.0;
+      }
+      solve1(r1);
+      solve2(r2, r3);
+      // size of the correction relative to the solution, in the error-norm
+      // weights
+      double cn = 0.0, xn = 0.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        b1[q] = x1[q] + r1[q];
+        b2[q] = x2[q] + r2[q];
+        b3[q] = x3[q] + r3[q];
+        if (RW::valid(ln, q)) {
+          const double w = sc[q] * sc[q];
+          cn += w * (r1[q] * r1[q] + r2[q] * r2[q] + r3[q] * r3[q]);
+          xn += w * (b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q]);
+        }
+      }
+      cn = Grp<G>::sum(cn);
+      xn = Grp<G>::sum(xn);
+      if (cn > 1e-2 * xn) bad = true;          // > 10 % correction
+      else if (cn <= 1e-12 * xn) refine = 0;   // <= 1e-6: plain FP32 solves from here on
+    }
+  }
+
   // RK_Decomp, radau2a.c:256-278: E1 = gamma/H - J, E2 = (alpha + i beta)/H - J,
   // then the real and (if E1 is regular) the complex LU.  Returns info.  The
   // complex factorization also runs when E1 is singular (its info is then
@@ -1002,6 +1143,7 @@ struct Radau {
       lu_real<N, G, R, MT>(E1, pos1, perm1, rd1, info);
       lu_cplx<N, G, R, MT>(E2r, E2i, pos2, perm2, rd2r, rd2i, info2);
     }
+    if constexpr (kMixed) refine = 1;
     return info != 0 ? info : info2;
   }
 
@@ -1023,6 +1165,7 @@ struct Radau {
     t = t_start;
     t_end = t_stop;
     Reject = false; FirstStep = true; SkipJac = false; SkipLU = false;
+    UseStart = false;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       sc[q] = 1.0 / (P.atol + fabs(y[q]) * P.rtol);  // scale_init, :88-94
@@ -1032,10 +1175,37 @@ struct Radau {
     }
     Nconsecutive = 0;
     Nsteps = 0;
+    if constexpr (kMixed) {
+      refine = 1;
+      bad = false;
+    }
     NewtonRate = 2.3784142300054421;  // pow(2.0, 1.25)
   }
 
   __device__ __forceinline__ bool finished() const { return !(t + kRoundoff < t_end); }
+
+  // Training sample of the learned start: features (y at the step start,
+  // log h) and the converged stages of an accepted first step.
+  __device__ __forceinline__ void log_first_step(const StartLog& L) const {
+    const int ln = Grp<G>::lane();
+    int slot = 0;
+    if (ln == 0) slot = atomicAdd(L.count, 1);
+    slot = Grp<G>::bcast_i(slot, 0);
+    if (slot >= L.cap) return;
+    float* x = L.x + (size_t)slot * (N + 1);
+    float* z = L.z + (size_t)slot * 3 * N;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (RW::valid(ln, q)) {
+        const int r = RW::row(ln, q);
+        x[r] = (float)y[q];
+        z[r] = (float)Z1[q];
+        z[N + r] = (float)Z2[q];
+        z[2 * N + r] = (float)Z3[q];
+      }
+    }
+    if (ln == 0) x[N] = (float)log(H);
+  }
 
   // One pass of the step loop, radau2a.c:566-734.  Returns RADAU_CONTINUE,
   // EC_success once t has reached t_end, or an error code.
@@ -1056,6 +1226,7 @@ struct Radau {
       cnt[CNT_LU]++;
       if (info != 0) {
         cnt[CNT_LU_FAIL]++;
+        UseStart = false;
         Nconsecutive += 1;
         if (Nconsecutive >= kMaxConsecutiveErrs) return EC_consecutive_steps;
         H *= 0.5;
@@ -1073,7 +1244,11 @@ struct Radau {
     if (0.1 * fabs(H) <= fabs(t) * kRoundoff) return EC_h_plus_t_equals_h;
     if (FirstStep) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) { Z1[q] = 0.0; Z2[q] = 0.0; Z3[q] = 0.0; }
+      for (int q = 0; q < R; ++q) {
+        Z1[q] = UseStart ? Zs1[q] : 0.0;
+        Z2[q] = UseStart ? Zs2[q] : 0.0;
+        Z3[q] = UseStart ? Zs3[q] : 0.0;
+      }
     } else {
       // RK_Interpolate, :302-314 (third stage keeps the reference's x2 factor)
       const double rr = H / Hold;
@@ -1122,8 +1297,13 @@ struct Radau {
           DZ2[q] = rkTiAi10 * x1 + rkTiAi11 * x2 + rkTiAi12 * x3;
           DZ3[q] = rkTiAi20 * x1 + rkTiAi21 * x2 + rkTiAi22 * x3;
         }
-        solve1(DZ1);
-        solve2(DZ2, DZ3);
+        if constexpr (kMixed) {
+        if (refine) solve_newton(ctx, DZ1, DZ2, DZ3);
+        else { solve1(DZ1); solve2(DZ2, DZ3); }
+        if (bad) return EC_needs_fp64;
+      } else {
+        solve_newton(ctx, DZ1, DZ2, DZ3);
+      }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const double x1 = DZ1[q], x2 = DZ2[q], x3 = DZ3[q];
@@ -1169,6 +1349,7 @@ struct Radau {
     }
     if (!NewtonDone) {
       cnt[CNT_NEWTON_FAIL]++;
+      UseStart = false;
       H = Fac * H;
       Reject = true;
       SkipJac = true;
@@ -1206,6 +1387,7 @@ struct Radau {
     double Hnew = Fac * H;
     if (Err < 1.0) {
       cnt[CNT_ACCEPT]++;
+      if (FirstStep && P.log.cap > 0) log_first_step(P.log);
       if (!FirstStep) {  // Gustafsson, :676-685
         double FacGus = kFacSafe * (H / Hacc) * pow(Err * Err / ErrOld, -0.25);
         FacGus = fmin(kFacMax, fmax(kFacMin, FacGus));
@@ -1251,6 +1433,7 @@ struct Radau {
       return finished() ? EC_success : RADAU_CONTINUE;
     } else {
       cnt[CNT_REJECT]++;
+      UseStart = false;
       if (FirstStep || Reject) {
         H = kFacRej * H;
       } else {

@@ -224,3 +224,30 @@ def test_kinetics54_tight_golden(session, net54, precision):
     print("kinetics54 precision %d vs tight: median %.3g max %.3g" % (precision, np.median(err), err.max()))
     assert np.median(err) <= 0.5
     assert err.max() <= 10.0
+
+
+def test_kinetics54_mixed_after_transient(session, oracle_lib, net54):
+    """FP32 factors after the first global step (the bench's timed regime), 4096
+    conditions, 5 global steps of 1e-6 s from the states FP64 reaches after one
+    step, against the FP64 oracle from the same states."""
+    num, steps, dt = 4096, 5, 1e-6
+    y0, P = M.initial_conditions(net54, num, seed=20261202 + 1, T_range=(800.0, 2200.0))
+    _kin_setup(net54, num)
+    y1 = y0.copy()
+    acc.integrate(num, 0.0, dt, dt, y1, P)
+    assert (acc.get_status(num) == 0).all()
+    acc.set_precision(acc.PRECISION_MIXED)
+    try:
+        y = y1.copy()
+        acc.integrate(num, dt, (steps + 1) * dt, dt, y, P)
+        codes = acc.get_status(num)
+        esc = acc.escalated()
+    finally:
+        acc.set_precision(acc.PRECISION_FP64)
+    om = oracle_lib.Mechanism.kinetics(net54)
+    yo, co, _ = oracle_lib.integrate_steps(om, y1, P, dt, steps, nthreads=8)
+    assert (codes == co).all()
+    we = weighted_err(y, yo).max(0)
+    print("kinetics54 mixed after transient: %d escalated; per-condition max weighted err median %.3g p99 %.3g max %.3g"
+          % (esc, np.median(we), np.percentile(we, 99), we.max()))
+    assert we.max() <= 10.0

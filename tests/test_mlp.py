@@ -1,0 +1,159 @@
+"""Learned Newton start: the network's kernels against an FP64 CPU restatement
+(tests/mlp_ref.py), and the trained network inside the integrator.
+
+* CPU: the restatement's backward pass against central differences (FP64, no
+  BF16 rounding), which pins the gradient formulas it shares with mlp.hip.
+* GPU: loss, gradient and batched forward pass of the BF16-MFMA kernels
+  against the restatement with the same BF16 operand rounding (norm-wise
+  relative error <= 2e-3: FP32 versus FP64 accumulation); the integrator's
+  per-wavefront inference path against the batched path; training reduces the
+  loss; and an A/B run on fresh conditions shows what the trained start changes
+  (Newton iterations, accepted steps) with both runs within 10 x tolerance of
+  the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import mlp_ref
+
+N54 = 54
+
+
+def _rand_params(net, rng, scale=0.3):
+    parts = []
+    for s in net.shapes:
+        a = rng.standard_normal(s) * (scale if len(s) == 2 else 0.1)
+        parts.append(a)
+    W1, b1, W2, b2, W3, b3 = parts
+    W1[net.inn:, :] = 0.0
+    W3[:, net.out:] = 0.0
+    b3[net.out:] = 0.0
+    return net.pack([W1, b1, W2, b2, W3, b3]).astype(np.float32)
+
+
+def test_reference_gradient_matches_central_differences():
+    rng = np.random.default_rng(5)
+    net = mlp_ref.Net(3)
+    p = _rand_params(net, rng).astype(np.float64)
+    x = rng.standard_normal((16, net.inn))
+    t = rng.standard_normal((16, net.out))
+    _, g = net.loss_grad(p, x, t, bf=False)
+    idx = rng.choice(np.flatnonzero(np.abs(g) > 0), 60, replace=False)
+    for i in idx:
+        h = 1e-6 * max(1.0, abs(p[i]))
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (net.loss(pp, x, t, bf=False) - net.loss(pm, x, t, bf=False)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 * abs(g[i]) + 1e-10, (i, fd, g[i])
+
+
+def test_bf16_rounding_is_round_to_nearest_even():
+    x = np.array([1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -2.5, 3.0e38, 1e-40], dtype=np.float32)
+    r = mlp_ref.bf16(x)
+    assert r[0] == 1.0
+    assert r[1] == 1.0                      # tie -> even
+    assert r[2] == 1.0 + 2 * 2.0 ** -7      # tie -> even (up)
+    assert r[3] == -2.5
+    assert np.isfinite(r[4])
+
+
+@pytest.fixture
+def accmod():
+    import accelerint_amd as acc
+    acc.lib()
+    yield acc
+    acc.lib().accelerInt_mlp_release()
+
+
+@pytest.mark.gpu
+def test_mlp_loss_gradient_against_cpu_restatement(accmod):
+    acc = accmod
+    rng = np.random.default_rng(11)
+    net = mlp_ref.Net(N54)
+    acc.mlp_init(N54, 3)
+    p = _rand_params(net, rng)
+    acc.mlp_set_params(p)
+    assert np.array_equal(acc.mlp_params(), p)
+    B = 64
+    x = rng.standard_normal((B, net.inn)).astype(np.float32)
+    t = rng.standard_normal((B, net.out)).astype(np.float32)
+    loss_g, g_g = acc.mlp_loss_grad(x, t)
+    loss_c, g_c = net.loss_grad(p.astype(np.float64), x.astype(np.float64), t.astype(np.float64), bf=True)
+    assert abs(loss_g - loss_c) <= 1e-4 * loss_c
+    o = 0
+    for s, name in zip(net.shapes, ["W1", "b1", "W2", "b2", "W3", "b3"]):
+        k = int(np.prod(s))
+        a, b = g_g[o:o + k].astype(np.float64), g_c[o:o + k]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print("grad %s: relative error %.2e" % (name, rel))
+        assert rel <= 2e-3, name
+        o += k
+    y_g = acc.mlp_predict(x, net.out)
+    y_c = net.predict(p.astype(np.float64), x.astype(np.float64), bf=True)
+    assert np.abs(y_g - y_c).max() <= 2e-3 * np.abs(y_c).max()
+
+
+@pytest.mark.gpu
+def test_mlp_in_kernel_inference_matches_batched(accmod):
+    acc = accmod
+    from accelerint_amd import mechanism as M
+    rng = np.random.default_rng(12)
+    net = mlp_ref.Net(N54)
+    acc.mlp_init(N54, 4)
+    p = _rand_params(net, rng)
+    acc.mlp_set_params(p)
+    mech = M.network(54)
+    y0, _ = M.initial_conditions(mech, 64, seed=3, T_range=(800.0, 2200.0))
+    h = 5e-7
+    feats = np.concatenate([y0.T, np.full((64, 1), np.log(h))], axis=1)
+    mu = feats.mean(0)
+    sd = np.where(feats.std(0) > 0, feats.std(0), 1.0)
+    zs = rng.uniform(0.5, 2.0, net.out)
+    acc.mlp_set_normalization(mu, sd, zs)
+    z_k = acc.mlp_predict_states(np.ascontiguousarray(y0.T), h)
+    x = ((feats - mu) / sd).astype(np.float32)
+    z_b = acc.mlp_predict(x, net.out).astype(np.float64) * zs
+    err = np.abs(z_k - z_b).max() / np.abs(z_b).max()
+    print("in-kernel vs batched inference: max relative difference %.2e" % err)
+    assert err <= 2e-2
+
+
+@pytest.mark.gpu
+def test_learned_start_ab(accmod, oracle_lib):
+    acc = accmod
+    from accelerint_amd import learned, mechanism as M
+    mech = M.network(54)
+    dt = 1e-6
+    acc.set_kinetics(mech)
+    acc.set_tolerances(1e-10, 1e-6)
+    acc.set_const_time_step(False)
+    acc.set_abort_on_error(False)
+    acc.initialize(8192, 0)
+    try:
+        y_tr, P_tr = M.initial_conditions(mech, 8192, seed=777, T_range=(800.0, 2200.0))
+        X, Z = learned.collect(y_tr, P_tr, 8, dt)
+        print("collected %d samples" % X.shape[0])
+        assert X.shape[0] >= 8192 * 7
+        hist = learned.train(X, Z, epochs=30, batch=512, lr=2e-3, seed=1)
+        print("training loss per epoch: first %.4g last %.4g" % (hist[0], hist[-1]))
+        assert hist[-1] < 0.5 * hist[0]
+        # A/B on conditions not seen in training
+        num, steps = 2048, 4
+        y0, P = M.initial_conditions(mech, num, seed=778, T_range=(800.0, 2200.0))
+        res = {}
+        for on in (False, True):
+            y, codes, cnt = learned.integrate_ab(y0, P, steps, dt, acc.PRECISION_MIXED, on)
+            res[on] = (y, codes, cnt)
+        yo, co, cnt_o = oracle_lib.integrate_steps(oracle_lib.Mechanism.kinetics(mech), y0, P, dt, steps, nthreads=8)
+        for on in (False, True):
+            y, codes, cnt = res[on]
+            assert (codes == co).all()
+            we = np.abs(y - yo) / (1e-10 + 1e-6 * np.abs(yo))
+            print("learned=%s: newton %d, steps %d, f %d, lu %d; max weighted err vs oracle %.3g"
+                  % (on, cnt[:, 6].sum(), cnt[:, 0].sum(), cnt[:, 7].sum(), cnt[:, 4].sum(), we.max()))
+            assert we.max() <= 10.0
+        gain = 1.0 - res[True][2][:, 6].sum() / res[False][2][:, 6].sum()
+        print("Newton iterations saved by the learned start: %.2f %%" % (100 * gain))
+    finally:
+        acc.cleanup()
