@@ -44,6 +44,8 @@ EXPORTS = (
     "accelerInt_mlp_set_params", "accelerInt_mlp_set_normalization", "accelerInt_mlp_loss_grad",
     "accelerInt_mlp_predict", "accelerInt_mlp_train", "accelerInt_mlp_last_error", "accelerInt_mlp_release",
     "accelerInt_set_learned_start", "accelerInt_log_first_steps", "accelerInt_mlp_predict_states",
+    "accelerInt_integrate_device_hetero", "accelerInt_mlp_train_host", "accelerInt_start_log",
+    "accelerInt_read_log",
 )
 
 
@@ -102,6 +104,10 @@ def lib():
         "accelerInt_set_learned_start": ([i], i),
         "accelerInt_log_first_steps": ([v, v, i, v], i),
         "accelerInt_mlp_predict_states": ([i, d, v, v], i),
+        "accelerInt_integrate_device_hetero": ([i, v, v, v, i, v, v, v, d, d, d, v], i),
+        "accelerInt_mlp_train_host": ([i, v, v, i, i, ctypes.c_float, ctypes.c_uint, v], i),
+        "accelerInt_start_log": ([i], i),
+        "accelerInt_read_log": ([v, v, i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -230,6 +236,27 @@ def mlp_predict_states(y_cond_major, h: float):
     return z
 
 
+def mlp_train_host(x, t, epochs: int, batch: int, lr: float, seed: int = 0):
+    """Adam on host arrays of normalized samples; returns the per-epoch loss."""
+    x, t = _f32(x), _f32(t)
+    hist = np.zeros(epochs, dtype=np.float32)
+    _mlp_check(lib().accelerInt_mlp_train_host(x.shape[0], x.ctypes.data, t.ctypes.data, int(epochs), int(batch),
+                                               float(lr), int(seed), hist.ctypes.data), "accelerInt_mlp_train_host")
+    return hist
+
+
+def start_log(capacity: int):
+    _check(lib().accelerInt_start_log(int(capacity)), "accelerInt_start_log")
+
+
+def read_log(capacity: int, n_states: int):
+    """(X, Z) of the library-owned log: X[i] = (y, log h), Z[i] = (Z1, Z2, Z3)."""
+    x = np.zeros((capacity, n_states + 1), dtype=np.float32)
+    z = np.zeros((capacity, 3 * n_states), dtype=np.float32)
+    n = _check(lib().accelerInt_read_log(x.ctypes.data, z.ctypes.data, int(capacity)), "accelerInt_read_log")
+    return x[:n], z[:n]
+
+
 def set_learned_start(on: bool):
     _check(lib().accelerInt_set_learned_start(int(bool(on))), "accelerInt_set_learned_start")
 
@@ -324,6 +351,17 @@ def integrate_device(NUM: int, t_start: float, t_end: float, stepsize: float, y_
                                              ctypes.c_void_p(codes_dev_ptr or None),
                                              ctypes.c_void_p(stream_ptr or None)),
            "accelerInt_integrate_device")
+
+
+def integrate_device_hetero(NUM14: int, y14_ptr: int, var14_ptr: int, codes14_ptr: int, NUM54: int,
+                            y54_ptr: int, var54_ptr: int, codes54_ptr: int, t_start: float, t_end: float,
+                            stepsize: float, stream_ptr: int = 0):
+    """14- and 54-state batches (device pointers, condition-major) in one persistent launch."""
+    vp = lambda p: ctypes.c_void_p(p or None)
+    _check(lib().accelerInt_integrate_device_hetero(int(NUM14), vp(y14_ptr), vp(var14_ptr), vp(codes14_ptr),
+                                                    int(NUM54), vp(y54_ptr), vp(var54_ptr), vp(codes54_ptr),
+                                                    float(t_start), float(t_end), float(stepsize), vp(stream_ptr)),
+           "accelerInt_integrate_device_hetero")
 
 
 def eval_callbacks(NUM: int, t: float, y_host: np.ndarray, var_host: np.ndarray):

@@ -56,12 +56,6 @@ __device__ __forceinline__ int grab_condition(int* work) {
   return Grp<G>::bcast_i(c, 0);
 }
 
-//
-// Escalation (mixed precision): with `esc` set, a condition that ends with a
-// non-zero code or a non-finite state is not written back; its index is
-// appended to esc[1..] (esc[0] = count) for an FP64 pass, which runs the same
-// kernel over that list (`list`, with its length read from `num_dev`) from the
-// untouched initial state.
 // Learned Newton start for the first step of a global step (one condition
 // per wavefront; the network's system size must match).
 template <class Mech, int G, int R, class St>
@@ -75,39 +69,53 @@ __device__ __forceinline__ void predict_start(const SolverParams& P, St& st) {
   }
 }
 
-template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
-    int num, double t_start, double t_end, double step, const double* __restrict__ pr,
-    double* __restrict__ y, int* __restrict__ codes, uint32_t* __restrict__ counters,
-    int* __restrict__ work, const int* __restrict__ list, const int* __restrict__ num_dev,
-    int* __restrict__ esc, SolverParams P, typename Mech::Params mp) {
-  extern __shared__ double lds_dyn[];
-  if (num_dev) num = *num_dev;
+// One batch of conditions of one mechanism, condition-major in HBM.
+//
+// Escalation (mixed precision): with `esc` set, a condition that ends with a
+// non-zero code or a non-finite state is not written back; its index is
+// appended to esc[1..] (esc[0] = count) for an FP64 pass, which runs the same
+// code over that list (`list`, with its length read from `num_dev`) from the
+// untouched initial state.
+struct Queue {
+  int num;
+  const double* pr;
+  double* y;
+  int* codes;
+  uint32_t* counters;
+  int* work;
+  const int* list;
+  const int* num_dev;
+  int* esc;
+};
+
+// The persistent loop of a lane group over a queue: pull a condition, run all
+// of its global steps back to back (one step attempt per iteration), write it
+// back, pull the next.
+template <class Mech, int G, int R, class MT>
+__device__ __forceinline__ void run_queue(const Queue& qa, double t_start, double t_end, double step,
+                                          const SolverParams& P, typename Mech::Ctx& ctx) {
   constexpr int N = Mech::N;
-  const int g = threadIdx.x / G;
-  const int ln = threadIdx.x % G;
-  using Integrator = Radau<Mech, G, R, MT>;
-  Mech::block_init(mp, lds_dyn);
-  typename Mech::Ctx ctx(mp, g, lds_dyn, Integrator::kNeedsScratch);
-  Integrator st;
+  const int ln = Grp<G>::lane();
+  const int num = qa.num_dev ? *qa.num_dev : qa.num;
+  Radau<Mech, G, R, MT> st;
   uint32_t cnt[NCNT];
   double prc = 0.0, t_next = t_end;
   int k = 0;
-  int ci = grab_condition<G>(work);
+  int ci = grab_condition<G>(qa.work);
   int c = 0;
   bool fresh = true;
 #pragma unroll 1
   while (ci < num) {
     if (fresh) {  // load condition c and open its first global step
-      c = list ? list[ci] : ci;
+      c = qa.list ? qa.list[ci] : ci;
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int r = ln + G * q;
-        st.y[q] = (r < N) ? y[(size_t)c * N + r] : 0.0;
+        st.y[q] = (r < N) ? qa.y[(size_t)c * N + r] : 0.0;
       }
 #pragma unroll
       for (int i = 0; i < NCNT; ++i) cnt[i] = 0;
-      prc = pr[c];
+      prc = qa.pr[c];
       k = 0;
       t_next = fmin(t_end, t_start + step);
       st.begin(P, t_start, t_next);
@@ -128,7 +136,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)
     // condition finished (all steps done, or an error code): write back, or
     // hand it to the FP64 pass
     bool escalate = false;
-    if (esc) {
+    if (qa.esc) {
       double bad = 0.0;
 #pragma unroll
       for (int q = 0; q < R; ++q)
@@ -139,19 +147,56 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE)
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int r = ln + G * q;
-        if (r < N) y[(size_t)c * N + r] = st.y[q];
+        if (r < N) qa.y[(size_t)c * N + r] = st.y[q];
       }
     }
     if (ln == 0) {
-      if (escalate) esc[1 + atomicAdd(esc, 1)] = c;
-      else if (codes) codes[c] = code;
-      if (counters) {
+      if (escalate) qa.esc[1 + atomicAdd(qa.esc, 1)] = c;
+      else if (qa.codes) qa.codes[c] = code;
+      if (qa.counters) {
 #pragma unroll
-        for (int i = 0; i < NCNT; ++i) counters[(size_t)c * NCNT + i] += cnt[i];
+        for (int i = 0; i < NCNT; ++i) qa.counters[(size_t)c * NCNT + i] += cnt[i];
       }
     }
-    ci = grab_condition<G>(work);
+    ci = grab_condition<G>(qa.work);
     fresh = true;
+  }
+}
+
+template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
+    Queue qa, double t_start, double t_end, double step, SolverParams P, typename Mech::Params mp) {
+  extern __shared__ double lds_dyn[];
+  const int g = threadIdx.x / G;
+  Mech::block_init(mp, lds_dyn);
+  typename Mech::Ctx ctx(mp, g, lds_dyn, Radau<Mech, G, R, MT>::kNeedsScratch);
+  run_queue<Mech, G, R, MT>(qa, t_start, t_end, step, P, ctx);
+}
+
+// Heterogeneous batch in one persistent launch: a 54-state and a 14-state
+// network.  Every wavefront first drains the 54-state queue (one condition per
+// wave), then the 14-state queue (four 16-lane groups per wave), so the long
+// jobs start first.  LDS: both networks' tables, then one region per wave
+// that serves the 54-state group and later the four 14-state groups.
+template <class MT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_radau_hetero(
+    Queue q54, Queue q14, double t_start, double t_end, double step, SolverParams P, KinParams mp54,
+    KinParams mp14, int off14, int off_regions, int region_doubles) {
+  extern __shared__ double lds_dyn[];
+  MechKinetics54::block_init(mp54, lds_dyn);
+  MechKinetics14::block_init(mp14, lds_dyn + off14);
+  const int w = threadIdx.x / 64;
+  double* region = lds_dyn + off_regions + (size_t)w * region_doubles;
+  {
+    MechKinetics54::Ctx ctx(mp54, lds_dyn, region, false);
+    run_queue<MechKinetics54, 64, 1, MT>(q54, t_start, t_end, step, P, ctx);
+  }
+  {
+    constexpr bool scratch = Radau<MechKinetics14, 16, 1, MT>::kNeedsScratch;
+    const int g = (threadIdx.x & 63) / 16;
+    MechKinetics14::Ctx ctx(mp14, lds_dyn + off14, region + (size_t)g * MechKinetics14::group_doubles(mp14.nr, scratch),
+                            scratch);
+    run_queue<MechKinetics14, 16, 1, MT>(q14, t_start, t_end, step, P, ctx);
   }
 }
 
@@ -291,8 +336,8 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
   int blocks = (num + GPB - 1) / GPB;
   if (blocks > resident) blocks = resident;
   hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work, esc);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, num, t_start, t_end, step, pr, y, codes,
-                     cnt, work, list, num_dev, esc, P, mp);
+  const Queue qa{num, pr, y, codes, cnt, work, list, num_dev, esc};
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, qa, t_start, t_end, step, P, mp);
   return hipGetLastError();
 }
 
@@ -308,6 +353,10 @@ struct State {
   int n = 2;
   int precision = ACCELERINT_PRECISION_FP64;
   int learned = 0;          // learned Newton start on (the network is looked up per launch)
+  float* own_log_x = nullptr;  // library-owned training-sample log (accelerInt_start_log)
+  float* own_log_z = nullptr;
+  int* own_log_n = nullptr;
+  int own_log_cap = 0;
   SolverParams P{1e-10, 1e-6, 0};
   int abort_on_error = 1;
   int device = -1;
@@ -319,11 +368,16 @@ struct State {
   int* d_codes = nullptr;
   uint32_t* d_cnt = nullptr;
   int* d_work = nullptr;    // atomic work counter of the persistent kernel
+  int* d_work2 = nullptr;   // second queue of a heterogeneous launch
+  int* d_esc2 = nullptr;    // escalation list of the second queue
   int* d_esc = nullptr;     // escalation list of the mixed-precision pass: [count, indices...]
   std::vector<int> h_codes;
   int have_status = 0;
   // kinetics tables (device copies)
-  KineticsHost kin;
+  // one kinetics network per compiled size (14 and 54 states); kin_n = the
+  // size of the primary one (the most recently installed)
+  KineticsHost kin14, kin54;
+  int kin_n = 0;
   MechKinetics14::Params kp14{};
   MechKinetics54::Params kp54{};
   std::string err;
@@ -364,6 +418,10 @@ void free_buffers() {
   if (S.d_cnt) hipFree(S.d_cnt);
   if (S.d_work) hipFree(S.d_work);
   if (S.d_esc) hipFree(S.d_esc);
+  if (S.d_work2) hipFree(S.d_work2);
+  if (S.d_esc2) hipFree(S.d_esc2);
+  S.d_work2 = nullptr;
+  S.d_esc2 = nullptr;
   S.d_work = nullptr;
   S.d_esc = nullptr;
   S.d_y = S.d_ycm = S.d_pr = nullptr;
@@ -382,6 +440,8 @@ bool alloc_buffers(int cap) {
   if (!HIPOK(hipMemset(S.d_cnt, 0, (size_t)cap * NCNT * sizeof(uint32_t)))) return false;
   if (!HIPOK(hipMalloc(&S.d_work, sizeof(int)))) return false;
   if (!HIPOK(hipMalloc(&S.d_esc, ((size_t)cap + 1) * sizeof(int)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_work2, sizeof(int)))) return false;
+  if (!HIPOK(hipMalloc(&S.d_esc2, ((size_t)cap + 1) * sizeof(int)))) return false;
   S.padded = cap;
   S.h_codes.assign(cap, 0);
   return true;
@@ -447,6 +507,71 @@ hipError_t launch_eval_any(int num, double t, const double* pr, const double* y,
   }
 }
 
+// Heterogeneous launch: the 14- and 54-state networks' batches in one
+// persistent kernel; with mixed precision, then the FP64 passes over each
+// batch's escalated conditions.
+template <class MT>
+hipError_t launch_hetero(int n14, const double* pr14, double* y14, int* codes14, int n54, const double* pr54,
+                         double* y54, int* codes54, double t_start, double t_end, double step, hipStream_t s) {
+  constexpr bool kMixed = std::is_same<MT, float>::value;
+  constexpr bool sc14 = Radau<MechKinetics14, 16, 1, MT>::kNeedsScratch;
+  auto kern = k_radau_hetero<MT>;
+  const int off14 = S.kp54.lds_bytes / 8;
+  const int off_regions = off14 + S.kp14.lds_bytes / 8;
+  const size_t r54 = MechKinetics54::group_doubles(S.kp54.nr, false);
+  const size_t r14 = 4 * MechKinetics14::group_doubles(S.kp14.nr, sc14);
+  const int region = (int)(r54 > r14 ? r54 : r14);
+  const size_t lds = ((size_t)off_regions + 4 * (size_t)region) * sizeof(double);
+  static int resident = 0;
+  static size_t lds_ok = 0;
+  if (resident == 0 || lds_ok != lds) {
+    int dev = 0, cus = 0, per_cu = 0, max_lds = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    if (lds > (size_t)max_lds) {
+      char buf[200];
+      snprintf(buf, sizeof buf, "heterogeneous launch needs %zu B of LDS per block, device allows %d", lds, max_lds);
+      g_launch_msg = buf;
+      return hipErrorInvalidConfiguration;
+    }
+    e = allow_lds(kern, lds);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu <= 0) {
+      g_launch_msg = "heterogeneous launch: no block fits on a compute unit";
+      return hipErrorInvalidConfiguration;
+    }
+    resident = (cus > 0 ? cus : 1) * per_cu;
+    lds_ok = lds;
+  }
+  int want = (n54 + 3) / 4 > (n14 + 15) / 16 ? (n54 + 3) / 4 : (n14 + 15) / 16;
+  int blocks = want < resident ? want : resident;
+  if (blocks < 1) return hipSuccess;
+  int* esc54 = kMixed ? S.d_esc : nullptr;
+  int* esc14 = kMixed ? S.d_esc2 : nullptr;
+  hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, S.d_work, esc54);
+  hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, S.d_work2, esc14);
+  const Queue q54{n54, pr54, y54, codes54, nullptr, S.d_work, nullptr, nullptr, esc54};
+  const Queue q14{n14, pr14, y14, codes14, nullptr, S.d_work2, nullptr, nullptr, esc14};
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, s, q54, q14, t_start, t_end, step, S.P, S.kp54, S.kp14,
+                     off14, off_regions, region);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !kMixed) return e;
+  if (n54 > 0) {
+    e = launch_radau<MechKinetics54, 64, 1, 256, ACC_K54_WPE, double>(n54, t_start, t_end, step, pr54, y54, codes54,
+                                                                      nullptr, S.d_work, S.d_esc + 1, S.d_esc,
+                                                                      nullptr, S.P, S.kp54, s);
+    if (e != hipSuccess) return e;
+  }
+  if (n14 > 0)
+    e = launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE, double>(n14, t_start, t_end, step, pr14, y14, codes14,
+                                                                      nullptr, S.d_work2, S.d_esc2 + 1, S.d_esc2,
+                                                                      nullptr, S.P, S.kp14, s);
+  return e;
+}
+
 // Global-step loop of solver_interface.cu:225-275 on device-resident,
 // condition-major buffers: one launch covers every global step.
 int run_steps(int num, double t_start, double t_end, double stepsize, double* y, const double* pr,
@@ -492,12 +617,12 @@ int accelerInt_select_mechanism(int kind) {
     return 0;
   }
   if (kind == ACCELERINT_MECH_KINETICS) {
-    if (S.kin.ns == 0) {
+    if (S.kin_n == 0) {
       set_err("accelerInt_set_kinetics must be called before selecting the kinetics mechanism");
       return -1;
     }
     S.mech = kind;
-    S.n = S.kin.ns + 1;
+    S.n = S.kin_n;
     return 0;
   }
   set_err("unknown mechanism kind %d", kind);
@@ -519,12 +644,14 @@ int accelerInt_set_kinetics(int ns, int nr, const double* W, const double* therm
     return -1;
   }
   std::string msg;
-  if (!S.kin.set(ns, nr, W, thermo, slots, lnA, beta, EaR, &msg)) {
+  KineticsHost& kin = (ns + 1 == 14) ? S.kin14 : S.kin54;
+  if (!kin.set(ns, nr, W, thermo, slots, lnA, beta, EaR, &msg)) {
     set_err("%s", msg.c_str());
     return -1;
   }
   S.mech = ACCELERINT_MECH_KINETICS;
   S.n = ns + 1;
+  S.kin_n = ns + 1;
   return 0;
 }
 
@@ -563,6 +690,52 @@ int accelerInt_set_learned_start(int on) {
   return 0;
 }
 
+int accelerInt_start_log(int capacity) {
+  if (!ensure_init()) return -1;
+  if (capacity <= 0) {
+    set_err("accelerInt_start_log: capacity must be positive");
+    return -1;
+  }
+  if (S.own_log_x) hipFree(S.own_log_x);
+  if (S.own_log_z) hipFree(S.own_log_z);
+  if (S.own_log_n) hipFree(S.own_log_n);
+  S.own_log_x = S.own_log_z = nullptr;
+  S.own_log_n = nullptr;
+  if (!HIPOK(hipMalloc(&S.own_log_x, (size_t)capacity * (S.n + 1) * sizeof(float))) ||
+      !HIPOK(hipMalloc(&S.own_log_z, (size_t)capacity * 3 * S.n * sizeof(float))) ||
+      !HIPOK(hipMalloc(&S.own_log_n, sizeof(int))) || !HIPOK(hipMemset(S.own_log_n, 0, sizeof(int))))
+    return -1;
+  S.own_log_cap = capacity;
+  S.P.log = StartLog{S.own_log_x, S.own_log_z, S.own_log_n, capacity};
+  return 0;
+}
+
+int accelerInt_read_log(float* x_host, float* z_host, int capacity) {
+  if (S.device < 0 || !S.own_log_n) {
+    set_err("accelerInt_read_log: no library-owned log (accelerInt_start_log)");
+    return -1;
+  }
+  int n = 0;
+  if (!HIPOK(hipStreamSynchronize(S.stream)) || !HIPOK(hipDeviceSynchronize()) ||
+      !HIPOK(hipMemcpy(&n, S.own_log_n, sizeof(int), hipMemcpyDeviceToHost)))
+    return -1;
+  if (n > S.own_log_cap) n = S.own_log_cap;
+  if (n > capacity) n = capacity;
+  if (n > 0 && ((x_host && !HIPOK(hipMemcpy(x_host, S.own_log_x, (size_t)n * (S.n + 1) * sizeof(float),
+                                             hipMemcpyDeviceToHost))) ||
+                (z_host && !HIPOK(hipMemcpy(z_host, S.own_log_z, (size_t)n * 3 * S.n * sizeof(float),
+                                             hipMemcpyDeviceToHost)))))
+    return -1;
+  hipFree(S.own_log_x);
+  hipFree(S.own_log_z);
+  hipFree(S.own_log_n);
+  S.own_log_x = S.own_log_z = nullptr;
+  S.own_log_n = nullptr;
+  S.own_log_cap = 0;
+  S.P.log = StartLog{nullptr, nullptr, nullptr, 0};
+  return n;
+}
+
 int accelerInt_log_first_steps(float* x_dev, float* z_dev, int capacity, int* count_dev) {
   if (capacity > 0 && (!x_dev || !z_dev || !count_dev)) {
     set_err("accelerInt_log_first_steps: null buffer");
@@ -597,16 +770,19 @@ void accelerInt_initialize(int NUM, int device) {
   if (!HIPOK(hipSetDevice(device))) return;
   if (S.mech == ACCELERINT_MECH_KINETICS) {
     std::string msg;
-    if (!S.kin.upload(&msg)) {
-      set_err("%s", msg.c_str());
-      if (S.abort_on_error) {
-        fprintf(stderr, "%s\n", msg.c_str());
-        exit(-1);
+    for (KineticsHost* kin : {&S.kin14, &S.kin54}) {
+      if (kin->ns == 0) continue;
+      if (!kin->upload(&msg)) {
+        set_err("%s", msg.c_str());
+        if (S.abort_on_error) {
+          fprintf(stderr, "%s\n", msg.c_str());
+          exit(-1);
+        }
+        return;
       }
-      return;
     }
-    S.kp14 = S.kin.params<14>();
-    S.kp54 = S.kin.params<54>();
+    S.kp14 = S.kin14.params<14>();
+    S.kp54 = S.kin54.params<54>();
   }
   if (!HIPOK(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking))) return;
   size_t free_mem = 0, total_mem = 0;
@@ -680,6 +856,43 @@ void intDriver(const int NUM, const double t, const double t_end, const double* 
   accelerInt_integrate(NUM, t, t_end, t_end - t, y_global, pr_global);
 }
 
+int accelerInt_integrate_device_hetero(int NUM14, double* y14_dev, const double* var14_dev, int* codes14_dev,
+                                       int NUM54, double* y54_dev, const double* var54_dev, int* codes54_dev,
+                                       double t_start, double t_end, double stepsize, void* stream) {
+  if (!ensure_init()) return -1;
+  if (S.mech != ACCELERINT_MECH_KINETICS || S.kin14.ns != 13 || S.kin54.ns != 53) {
+    set_err("accelerInt_integrate_device_hetero: install a 14-state and a 54-state network first");
+    return -2;
+  }
+  if (NUM14 < 0 || NUM54 < 0 || NUM14 > S.padded || NUM54 > S.padded || (NUM14 > 0 && (!y14_dev || !var14_dev)) ||
+      (NUM54 > 0 && (!y54_dev || !var54_dev))) {
+    set_err("accelerInt_integrate_device_hetero: bad arguments (each batch at most %d conditions)", S.padded);
+    return -2;
+  }
+  const double step = stepsize < 0 ? t_end - t_start : stepsize;
+  if (!(t_start + kRoundoff < t_end) || NUM14 + NUM54 == 0) return 0;
+  hipStream_t s = stream ? (hipStream_t)stream : S.stream;
+  g_launch_msg.clear();
+  S.P.mlp = MlpView{};
+  if (S.learned) {
+    S.P.mlp = mlp_view();
+    if (!S.P.mlp.on || S.P.mlp.n != 54) {
+      set_err("learned start is on but no 54-state network is loaded");
+      return -1;
+    }
+  }
+  hipError_t e = S.precision == ACCELERINT_PRECISION_MIXED
+                     ? launch_hetero<float>(NUM14, var14_dev, y14_dev, codes14_dev, NUM54, var54_dev, y54_dev,
+                                            codes54_dev, t_start, t_end, step, s)
+                     : launch_hetero<double>(NUM14, var14_dev, y14_dev, codes14_dev, NUM54, var54_dev, y54_dev,
+                                             codes54_dev, t_start, t_end, step, s);
+  if (e != hipSuccess && !g_launch_msg.empty()) {
+    set_err("%s", g_launch_msg.c_str());
+    return -1;
+  }
+  return hip_ok(e, "k_radau_hetero launch") ? 0 : -1;
+}
+
 int accelerInt_integrate_device(int NUM, double t_start, double t_end, double stepsize,
                                 double* y_dev, const double* var_dev, int* codes_dev,
                                 void* stream) {
@@ -748,7 +961,8 @@ void accelerInt_cleanup(void) {
   if (S.device < 0) return;
   hipStreamSynchronize(S.stream);
   free_buffers();
-  S.kin.release();
+  S.kin14.release();
+  S.kin54.release();
   hipStreamDestroy(S.stream);
   S.stream = nullptr;
   S.device = -1;

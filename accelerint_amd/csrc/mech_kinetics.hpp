@@ -114,8 +114,12 @@ struct MechKinetics {
     double* Jc;
     double* scratch;
     double* jac;
-    __device__ Ctx(const Params& pp, int g, double* lds, bool with_scratch) : nr(pp.nr) {
-      const char* t = reinterpret_cast<const char*>(lds);
+    // tables copied by block_init at `lds`; group g's buffers after them
+    __device__ Ctx(const Params& pp, int g, double* lds, bool with_scratch)
+        : Ctx(pp, lds, lds + pp.lds_bytes / 8 + (size_t)g * group_doubles(pp.nr, with_scratch), with_scratch) {}
+    // tables at `tables`, the group's buffers at `group` (group_doubles(nr, with_scratch) doubles)
+    __device__ Ctx(const Params& pp, const double* tables, double* group, bool with_scratch) : nr(pp.nr) {
+      const char* t = reinterpret_cast<const char*>(tables);
       invW = reinterpret_cast<const double*>(t + pp.off_invW);
       W = reinterpret_cast<const double*>(t + pp.off_W);
       thermo = reinterpret_cast<const double*>(t + pp.off_th);
@@ -123,7 +127,7 @@ struct MechKinetics {
       slots = reinterpret_cast<const unsigned long long*>(t + pp.off_sl);
       csr_off = reinterpret_cast<const int*>(t + pp.off_co);
       csr = reinterpret_cast<const int*>(t + pp.off_ce);
-      double* b = lds + pp.lds_bytes / 8 + (size_t)g * group_doubles(pp.nr, with_scratch);
+      double* b = group;
       C = b; b += NS;
       gRT = b; b += NS;
       hR = b; b += NS;

@@ -19,6 +19,7 @@ struct MechVdP {
     double* jac = nullptr;
     double* scratch = nullptr;
     __device__ Ctx(const Params&, int /*group*/, double* /*lds*/, bool /*scratch*/) {}
+    __device__ Ctx(const Params&, const double* /*tables*/, double* /*group*/, bool /*scratch*/) {}
   };
 
   template <int G, int R>

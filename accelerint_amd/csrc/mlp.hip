@@ -515,6 +515,23 @@ int accelerInt_mlp_predict_states(int num, double h, const double* y, double* z)
   return rc;
 }
 
+int accelerInt_mlp_train_host(int n, const float* x, const float* t, int epochs, int batch, float lr, unsigned seed,
+                              float* loss_per_epoch) {
+  if (!G.n || n <= 0 || !x || !t) {
+    G.err = "accelerInt_mlp_train_host: no network or empty data";
+    return -1;
+  }
+  float *dx = nullptr, *dt = nullptr;
+  int rc = -1;
+  if (ok(hipMalloc(&dx, (size_t)n * G.in * 4), "hipMalloc") && ok(hipMalloc(&dt, (size_t)n * G.out * 4), "hipMalloc") &&
+      ok(hipMemcpy(dx, x, (size_t)n * G.in * 4, hipMemcpyHostToDevice), "hipMemcpy") &&
+      ok(hipMemcpy(dt, t, (size_t)n * G.out * 4, hipMemcpyHostToDevice), "hipMemcpy"))
+    rc = accelerInt_mlp_train(n, dx, dt, epochs, batch, lr, seed, loss_per_epoch);
+  if (dx) hipFree(dx);
+  if (dt) hipFree(dt);
+  return rc;
+}
+
 const char* accelerInt_mlp_last_error(void) { return G.err.c_str(); }
 
 void accelerInt_mlp_release(void) { free_all(); }

@@ -876,10 +876,6 @@ __device__ __forceinline__ void rsolve_cplx(const double (&ar)[N], const double 
 // interleave step attempts of many conditions and refill a lane group with a
 // new condition or a new global step as soon as its current one ends.
 enum : int { RADAU_CONTINUE = -1 };
-// Internal code of the FP32-factor method: the factorization is too
-// inaccurate for its refined solves to be trusted; the condition goes to the
-// FP64 pass (never returned to callers).
-enum : int { EC_needs_fp64 = 64 };
 
 template <class Mech, int G, int R, class MT = double>
 struct Radau {
@@ -915,11 +911,6 @@ struct Radau {
   // stiff kinetics reach far beyond the FP32 range (reverse rate constants at
   // trial temperatures), and row scaling changes no solution
   int ex1[R], ex2[R];
-  // mixed precision: refine every Newton solve (1) or not (0); set to 1 by
-  // each factorization, cleared once a refined solve shows the FP32 factors
-  // accurate to 1e-6; a correction above 10 % of the solution flags `bad`
-  int refine;
-  bool bad;
   MT rd1[R], rd2r[R], rd2i[R];
   Perm<N> perm1, perm2;
   int src1, src2;  // column-LU path: original equation of this lane's row
@@ -1004,79 +995,6 @@ struct Radau {
     }
   }
 
-  // Newton solves (RK_Solve's two linear systems).  FP64 factors: one solve
-  // each.  FP32 factors: each solve is followed by one step of iterative
-  // refinement against the FP64 matrices gamma/H - J and (alpha + i beta)/H - J
-  // (the three residuals share one pass over the Jacobian's rows), so the
-  // simplified Newton iteration sees FP64-accurate increments and behaves as
-  // with FP64 factors whenever cond * 2^-24 << 1.  Without it, an iterate of
-  // an ill-conditioned system can pass the convergence test (whose rate comes
-  // from the previous step) with a sizeable error.
-  __device__ __forceinline__ void solve_newton(Ctx& ctx, double (&b1)[R], double (&b2)[R], double (&b3)[R]) {
-    if constexpr (!kMixed) {
-      solve1(b1);
-      solve2(b2, b3);
-    } else {
-      const int ln = Grp<G>::lane();
-      double x1[R], x2[R], x3[R];
-#pragma unroll
-      for (int q = 0; q < R; ++q) { x1[q] = b1[q]; x2[q] = b2[q]; x3[q] = b3[q]; }
-      solve1(x1);
-      solve2(x2, x3);
-      if constexpr (Mech::kJacInLds) group_lds_sync();
-      const double g1 = rkGamma / H, ar = rkAlpha / H, ai = rkBeta / H;
-      double r1[R], r2[R], r3[R];
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        r1[q] = b1[q] - g1 * x1[q];
-        r2[q] = b2[q] - (ar * x2[q] - ai * x3[q]);
-        r3[q] = b3[q] - (ar * x3[q] + ai * x2[q]);
-      }
-      static_for<0, N>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        const double xj1 = bcast_k<G, j % G>(x1[j / G]);
-        const double xj2 = bcast_k<G, j % G>(x2[j / G]);
-        const double xj3 = bcast_k<G, j % G>(x3[j / G]);
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-          double Jrj;
-          if constexpr (Mech::kJacInLds) Jrj = RW::valid(ln, q) ? ctx.jac[RW::row(ln, q) * N + j] : 0.0;
-          else Jrj = A[q][j];
-          r1[q] = fma(Jrj, xj1, r1[q]);
-          r2[q] = fma(Jrj, xj2, r2[q]);
-          r3[q] = fma(Jrj, xj3, r3[q]);
-        }
-      });
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const bool v = RW::valid(ln, q);
-        r1[q] = v ? r1[q] : 0.0;
-        r2[q] = v ? r2[q] : 0.0;
-        r3[q] = v ? r3[q] : 0.0;
-      }
-      solve1(r1);
-      solve2(r2, r3);
-      // size of the correction relative to the solution, in the error-norm
-      // weights
-      double cn = 0.0, xn = 0.0;
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        b1[q] = x1[q] + r1[q];
-        b2[q] = x2[q] + r2[q];
-        b3[q] = x3[q] + r3[q];
-        if (RW::valid(ln, q)) {
-          const double w = sc[q] * sc[q];
-          cn += w * (r1[q] * r1[q] + r2[q] * r2[q] + r3[q] * r3[q]);
-          xn += w * (b1[q] * b1[q] + b2[q] * b2[q] + b3[q] * b3[q]);
-        }
-      }
-      cn = Grp<G>::sum(cn);
-      xn = Grp<G>::sum(xn);
-      if (cn > 1e-2 * xn) bad = true;          // > 10 % correction
-      else if (cn <= 1e-12 * xn) refine = 0;   // <= 1e-6: plain FP32 solves from here on
-    }
-  }
-
   // RK_Decomp, radau2a.c:256-278: E1 = gamma/H - J, E2 = (alpha + i beta)/H - J,
   // then the real and (if E1 is regular) the complex LU.  Returns info.  The
   // complex factorization also runs when E1 is singular (its info is then
@@ -1143,7 +1061,6 @@ struct Radau {
       lu_real<N, G, R, MT>(E1, pos1, perm1, rd1, info);
       lu_cplx<N, G, R, MT>(E2r, E2i, pos2, perm2, rd2r, rd2i, info2);
     }
-    if constexpr (kMixed) refine = 1;
     return info != 0 ? info : info2;
   }
 
@@ -1175,10 +1092,7 @@ struct Radau {
     }
     Nconsecutive = 0;
     Nsteps = 0;
-    if constexpr (kMixed) {
-      refine = 1;
-      bad = false;
-    }
+
     NewtonRate = 2.3784142300054421;  // pow(2.0, 1.25)
   }
 
@@ -1297,13 +1211,8 @@ struct Radau {
           DZ2[q] = rkTiAi10 * x1 + rkTiAi11 * x2 + rkTiAi12 * x3;
           DZ3[q] = rkTiAi20 * x1 + rkTiAi21 * x2 + rkTiAi22 * x3;
         }
-        if constexpr (kMixed) {
-        if (refine) solve_newton(ctx, DZ1, DZ2, DZ3);
-        else { solve1(DZ1); solve2(DZ2, DZ3); }
-        if (bad) return EC_needs_fp64;
-      } else {
-        solve_newton(ctx, DZ1, DZ2, DZ3);
-      }
+          solve1(DZ1);
+        solve2(DZ2, DZ3);
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const double x1 = DZ1[q], x2 = DZ2[q], x3 = DZ3[q];
@@ -1336,7 +1245,11 @@ struct Radau {
       NewtonIncrementOld = fmax(NewtonIncrement, kRoundoff);
 #pragma unroll
       for (int q = 0; q < R; ++q) { Z1[q] -= DZ1[q]; Z2[q] -= DZ2[q]; Z3[q] -= DZ3[q]; }
-      NewtonDone = (NewtonRate * NewtonIncrement <= kNewtonTol);
+      // With FP32 factors the iteration is only linearly convergent, at a rate
+      // that depends on the factors' accuracy: convergence is declared only
+      // on a contraction rate measured in this step (from the second
+      // iteration on), never on the rate carried over from the previous step.
+      NewtonDone = (!kMixed || NewtonIter > 0) && (NewtonRate * NewtonIncrement <= kNewtonTol);
       if (NewtonDone) break;
       if (NewtonIter == kNewtonMaxit - 1) return EC_newton_max_iterations_exceeded;
     }

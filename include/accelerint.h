@@ -76,6 +76,9 @@ int accelerInt_select_mechanism(int kind);
  * cp/R = a0 + a1 T + a2 T^2, h/R = a0 T + a1 T^2/2 + a2 T^3/3 + bh,
  * s/R = a0 ln T + a1 T + a2 T^2/2 + bs; rate constants
  * kf = exp(lnA + beta ln(T/1000) - EaR/T) in mol-cm-s units.
+ * One network per compiled size is kept (installing a 14-state and a 54-state
+ * network enables accelerInt_integrate_device_hetero); the most recently
+ * installed one is the mechanism the other calls integrate.
  * Returns 0, or -1 if ns + 1 is not a compiled system size. */
 int accelerInt_set_kinetics(int ns, int nr, const double *W, const double *thermo,
                             const int *slots, const double *lnA, const double *beta,
@@ -121,6 +124,16 @@ int accelerInt_num_states(void);
 int accelerInt_integrate_device(int NUM, double t_start, double t_end, double stepsize,
                                 double *y_dev, const double *var_dev, int *codes_dev,
                                 void *stream);
+
+/* Heterogeneous batch in one persistent launch (no reference counterpart):
+ * conditions of the installed 14-state network (y14_dev[c*14 + i]) and of the
+ * installed 54-state network (y54_dev[c*54 + i]) integrated together, long jobs
+ * first; each batch at most NUM-at-initialize conditions; codes arrays may be
+ * NULL.  Precision, tolerances and the learned start (54-state part) as set.
+ * Asynchronous on `stream`.  Returns 0 or a negative error. */
+int accelerInt_integrate_device_hetero(int NUM14, double *y14_dev, const double *var14_dev, int *codes14_dev,
+                                       int NUM54, double *y54_dev, const double *var54_dev, int *codes54_dev,
+                                       double t_start, double t_end, double stepsize, void *stream);
 
 /* Evaluates the mechanism callbacks on the device at each condition's state
  * without integrating (the reference's rate/Jacobian dump,
@@ -168,6 +181,9 @@ int accelerInt_mlp_train(int n, const float *x_dev, const float *t_dev, int epoc
  * states: y[num][54] condition-major, step size h -> z[num][3*54] = (Z1, Z2, Z3)
  * denormalized.  54-state networks only. */
 int accelerInt_mlp_predict_states(int num, double h, const double *y, double *z);
+/* accelerInt_mlp_train on host arrays (copied to the device for the call). */
+int accelerInt_mlp_train_host(int n, const float *x, const float *t, int epochs, int batch, float lr,
+                              unsigned seed, float *loss_per_epoch);
 const char *accelerInt_mlp_last_error(void);
 void accelerInt_mlp_release(void);
 /* Use (1) or stop using (0) the trained network as the Newton start of each
@@ -179,6 +195,12 @@ int accelerInt_set_learned_start(int on);
  * (device int, zeroed by the caller) counts appended samples (it may exceed
  * capacity; samples beyond it are dropped).  capacity 0 turns logging off. */
 int accelerInt_log_first_steps(float *x_dev, float *z_dev, int capacity, int *count_dev);
+/* The same log in library-owned device buffers: start_log allocates room for
+ * `capacity` samples and turns logging on; read_log copies up to `capacity`
+ * samples to the host arrays, frees the buffers, turns logging off and returns
+ * the number of samples copied (negative on error). */
+int accelerInt_start_log(int capacity);
+int accelerInt_read_log(float *x_host, float *z_host, int capacity);
 
 /* Blocks until the library stream is idle; returns 0 or a negative error. */
 int accelerInt_synchronize(void);

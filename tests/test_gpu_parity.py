@@ -251,3 +251,37 @@ def test_kinetics54_mixed_after_transient(session, oracle_lib, net54):
     print("kinetics54 mixed after transient: %d escalated; per-condition max weighted err median %.3g p99 %.3g max %.3g"
           % (esc, np.median(we), np.percentile(we, 99), we.max()))
     assert we.max() <= 10.0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_heterogeneous_launch(session, oracle_lib, net14, net54, precision):
+    """A 14-state and a 54-state batch in one persistent launch, against the
+    oracle for each network."""
+    from hipmem import DeviceArray
+    n14, n54, steps, dt = 1024, 256, 3, 1e-6
+    y14, P14 = M.initial_conditions(net14, n14, seed=31)
+    y54, P54 = M.initial_conditions(net54, n54, seed=32, T_range=(800.0, 2200.0))
+    acc.set_kinetics(net14)
+    acc.set_kinetics(net54)
+    acc.set_tolerances(ATOL, RTOL)
+    acc.set_const_time_step(False)
+    acc.set_abort_on_error(False)
+    acc.initialize(max(n14, n54), 0)
+    acc.set_precision(precision)
+    d14, d54 = DeviceArray(y14.T), DeviceArray(y54.T)
+    p14, p54 = DeviceArray(P14), DeviceArray(P54)
+    c14, c54 = DeviceArray(np.full(n14, -1, np.int32)), DeviceArray(np.full(n54, -1, np.int32))
+    try:
+        acc.integrate_device_hetero(n14, d14.addr, p14.addr, c14.addr, n54, d54.addr, p54.addr, c54.addr,
+                                    0.0, steps * dt, dt)
+        acc.synchronize()
+    finally:
+        acc.set_precision(acc.PRECISION_FP64)
+    for mech, y0, P, d, c, label in ((net14, y14, P14, d14, c14, "14"), (net54, y54, P54, d54, c54, "54")):
+        y = d.get().T
+        codes = c.get()
+        yo, co, _ = oracle_lib.integrate_steps(oracle_lib.Mechanism.kinetics(mech), y0, P, dt, steps, nthreads=8)
+        assert (codes == co).all(), label
+        we = weighted_err(y, yo)
+        print("heterogeneous precision %d, %s-state part: max weighted err %.3g" % (precision, label, we.max()))
+        assert we.max() <= 10.0
