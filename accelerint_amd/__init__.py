@@ -45,7 +45,7 @@ EXPORTS = (
     "accelerInt_mlp_predict", "accelerInt_mlp_train", "accelerInt_mlp_last_error", "accelerInt_mlp_release",
     "accelerInt_set_learned_start", "accelerInt_log_first_steps", "accelerInt_mlp_predict_states",
     "accelerInt_integrate_device_hetero", "accelerInt_mlp_train_host", "accelerInt_start_log",
-    "accelerInt_read_log",
+    "accelerInt_read_log", "accelerInt_graph_capture", "accelerInt_graph_launch", "accelerInt_graph_release",
 )
 
 
@@ -108,6 +108,9 @@ def lib():
         "accelerInt_mlp_train_host": ([i, v, v, i, i, ctypes.c_float, ctypes.c_uint, v], i),
         "accelerInt_start_log": ([i], i),
         "accelerInt_read_log": ([v, v, i], i),
+        "accelerInt_graph_capture": ([i, d, d, d, v, v, v, v], i),
+        "accelerInt_graph_launch": ([i, v], i),
+        "accelerInt_graph_release": ([], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -362,6 +365,23 @@ def integrate_device_hetero(NUM14: int, y14_ptr: int, var14_ptr: int, codes14_pt
                                                     int(NUM54), vp(y54_ptr), vp(var54_ptr), vp(codes54_ptr),
                                                     float(t_start), float(t_end), float(stepsize), vp(stream_ptr)),
            "accelerInt_integrate_device_hetero")
+
+
+def graph_capture(NUM: int, t_start: float, t_end: float, stepsize: float, y_dev_ptr: int, var_dev_ptr: int,
+                  codes_dev_ptr: int = 0, stream_ptr: int = 0):
+    """Capture one device-resident integration as a hipGraph (see the C header)."""
+    vp = lambda p: ctypes.c_void_p(p or None)
+    _check(lib().accelerInt_graph_capture(int(NUM), float(t_start), float(t_end), float(stepsize), vp(y_dev_ptr),
+                                          vp(var_dev_ptr), vp(codes_dev_ptr), vp(stream_ptr)),
+           "accelerInt_graph_capture")
+
+
+def graph_launch(repeats: int, stream_ptr: int = 0):
+    _check(lib().accelerInt_graph_launch(int(repeats), ctypes.c_void_p(stream_ptr or None)), "accelerInt_graph_launch")
+
+
+def graph_release():
+    lib().accelerInt_graph_release()
 
 
 def eval_callbacks(NUM: int, t: float, y_host: np.ndarray, var_host: np.ndarray):

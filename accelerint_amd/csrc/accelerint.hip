@@ -305,7 +305,6 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
                         const typename Mech::Params& mp, hipStream_t s) {
   constexpr int GPB = BLOCK / G;
   auto kern = k_radau<Mech, G, R, BLOCK, WPE, MT>;
-  if (num <= 0) return hipSuccess;
   const size_t lds = Mech::lds_bytes(mp, GPB, Radau<Mech, G, R, MT>::kNeedsScratch);
   static int resident = 0;     // blocks per device that fit at once, per instantiation
   static size_t lds_ok = 0;    // LDS size the residency was computed for
@@ -333,6 +332,7 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
     resident = (cus > 0 ? cus : 1) * per_cu;
     lds_ok = lds;
   }
+  if (num <= 0) return hipSuccess;  // (num = 0: only the occupancy queries above)
   int blocks = (num + GPB - 1) / GPB;
   if (blocks > resident) blocks = resident;
   hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work, esc);
@@ -357,6 +357,8 @@ struct State {
   float* own_log_z = nullptr;
   int* own_log_n = nullptr;
   int own_log_cap = 0;
+  hipGraph_t graph = nullptr;          // captured integration (accelerInt_graph_capture)
+  hipGraphExec_t graph_exec = nullptr;
   SolverParams P{1e-10, 1e-6, 0};
   int abort_on_error = 1;
   int device = -1;
@@ -952,6 +954,63 @@ int accelerInt_escalated(void) {
   return n;
 }
 
+// hipGraph of one device-resident integration (all of its kernel launches:
+// work-counter resets, the persistent kernel, the FP64 escalation pass).  For
+// autonomous right-hand sides such as the kinetics networks, each replay
+// advances the states by the captured interval, so repeated global steps run
+// as graph launches with no per-step host work.
+int accelerInt_graph_capture(int NUM, double t_start, double t_end, double stepsize, double* y_dev,
+                             const double* var_dev, int* codes_dev, void* stream) {
+  if (!ensure_init()) return -1;
+  if (NUM <= 0 || NUM > S.padded || !y_dev || !var_dev) {
+    set_err("accelerInt_graph_capture: bad arguments (NUM must be 1..%d)", S.padded);
+    return -2;
+  }
+  if (S.graph_exec) hipGraphExecDestroy(S.graph_exec);
+  if (S.graph) hipGraphDestroy(S.graph);
+  S.graph_exec = nullptr;
+  S.graph = nullptr;
+  hipStream_t s = stream ? (hipStream_t)stream : S.stream;
+  // the occupancy queries run outside the capture
+  if (!hip_ok(launch(0, t_start, t_end, stepsize, var_dev, y_dev, codes_dev, nullptr, s), "occupancy query"))
+    return -1;
+  if (!HIPOK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed))) return -1;
+  const int rc = run_steps(NUM, t_start, t_end, stepsize, y_dev, var_dev, codes_dev, nullptr, s);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc != 0 || !HIPOK(e)) {
+    if (g) hipGraphDestroy(g);
+    return -1;
+  }
+  S.graph = g;
+  if (!HIPOK(hipGraphInstantiate(&S.graph_exec, S.graph, nullptr, nullptr, 0))) {
+    hipGraphDestroy(S.graph);
+    S.graph = nullptr;
+    S.graph_exec = nullptr;
+    return -1;
+  }
+  return 0;
+}
+
+int accelerInt_graph_launch(int repeats, void* stream) {
+  if (!S.graph_exec || repeats < 0) {
+    set_err("accelerInt_graph_launch: no captured graph");
+    return -1;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : S.stream;
+  for (int r = 0; r < repeats; ++r)
+    if (!HIPOK(hipGraphLaunch(S.graph_exec, s))) return -1;
+  return 0;
+}
+
+int accelerInt_graph_release(void) {
+  if (S.graph_exec) hipGraphExecDestroy(S.graph_exec);
+  if (S.graph) hipGraphDestroy(S.graph);
+  S.graph_exec = nullptr;
+  S.graph = nullptr;
+  return 0;
+}
+
 int accelerInt_synchronize(void) {
   if (S.device < 0) return 0;
   return HIPOK(hipStreamSynchronize(S.stream)) ? 0 : -1;
@@ -960,6 +1019,7 @@ int accelerInt_synchronize(void) {
 void accelerInt_cleanup(void) {
   if (S.device < 0) return;
   hipStreamSynchronize(S.stream);
+  accelerInt_graph_release();
   free_buffers();
   S.kin14.release();
   S.kin54.release();

@@ -251,23 +251,6 @@ __device__ __forceinline__ double dpp_t(double v) {
   return __hiloint2double(dpp_i<CTRL>(__double2hiint(v)), dpp_i<CTRL>(__double2loint(v)));
 }
 constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
-// row_bcast:15 / row_bcast:31 (lane 15 of each row into the next row; lane 31
-// into rows 2 and 3), used with a row mask that leaves the other rows unchanged.
-constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
-template <int CTRL, int ROWS>
-__device__ __forceinline__ int dpp_rows_i(int old, int v) {
-  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_rows_t(float old, float v) {
-  return __int_as_float(dpp_rows_i<CTRL, ROWS>(__float_as_int(old), __float_as_int(v)));
-}
-template <int CTRL, int ROWS>
-__device__ __forceinline__ double dpp_rows_t(double old, double v) {
-  return __hiloint2double(dpp_rows_i<CTRL, ROWS>(__double2hiint(old), __double2hiint(v)),
-                          dpp_rows_i<CTRL, ROWS>(__double2loint(old), __double2loint(v)));
-}
-
 // (value, key) order of the pivot search: larger value, then smaller key.
 // Bitwise, not short-circuit, so the exchange stays branch-free.
 template <class T>
@@ -290,20 +273,18 @@ __device__ __forceinline__ void group_argmax(T& bv, int& bkey) {
     argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
   } else if constexpr (G == 64) {
-    // DPP within each 16-lane row, then the four row winners by row broadcasts
-    // (kept in VGPRs; only the final winner is read out: merging the four row
-    // winners after reading them into scalar registers gave wrong pivots on
-    // gfx950)
+    // DPP within each 16-lane row, then the row winners by two xor exchanges.
+    // (Two other ways to join the rows picked wrong pivots on gfx950 and are
+    // not used: merging the four row winners after reading them into scalar
+    // registers, and DPP row_bcast:15 / row_bcast:31 with row masks.)
     argmax_merge(bv, bkey, dpp_t<kDppXor1>(bv), dpp_i<kDppXor1>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppXor2>(bv), dpp_i<kDppXor2>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
     // rows 1 and 3 take the winners of rows 0 and 2, then rows 2 and 3 the
     // winner of rows 0-1; lane 63 ends with the wave's winner
-    argmax_merge(bv, bkey, dpp_rows_t<kDppRowBcast15, 0xA>(bv, bv), dpp_rows_i<kDppRowBcast15, 0xA>(bkey, bkey));
-    argmax_merge(bv, bkey, dpp_rows_t<kDppRowBcast31, 0xC>(bv, bv), dpp_rows_i<kDppRowBcast31, 0xC>(bkey, bkey));
-    bkey = __builtin_amdgcn_readlane(bkey, 63);
-    bv = Grp<G>::bcast(bv, 63);
+    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 16), Grp<G>::xor_i(bkey, 16));
+    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 32), Grp<G>::xor_i(bkey, 32));
   } else {
 #pragma unroll
     for (int m = 1; m < G; m <<= 1) argmax_merge(bv, bkey, Grp<G>::xor_(bv, m), Grp<G>::xor_i(bkey, m));

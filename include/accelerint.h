@@ -202,6 +202,18 @@ int accelerInt_log_first_steps(float *x_dev, float *z_dev, int capacity, int *co
 int accelerInt_start_log(int capacity);
 int accelerInt_read_log(float *x_host, float *z_host, int capacity);
 
+/* hipGraph of one accelerInt_integrate_device call (no reference
+ * counterpart): capture records every kernel launch of the call for these
+ * buffers and interval; launch replays it `repeats` times on `stream` (NULL =
+ * the library's stream).  With autonomous right-hand sides (the kinetics
+ * networks) each replay advances the states by t_end - t_start, so a loop of
+ * global steps becomes one host call.  NUM at most NUM-at-initialize.  One
+ * graph at a time; capture replaces it, release (or cleanup) frees it. */
+int accelerInt_graph_capture(int NUM, double t_start, double t_end, double stepsize, double *y_dev,
+                             const double *var_dev, int *codes_dev, void *stream);
+int accelerInt_graph_launch(int repeats, void *stream);
+int accelerInt_graph_release(void);
+
 /* Blocks until the library stream is idle; returns 0 or a negative error. */
 int accelerInt_synchronize(void);
 

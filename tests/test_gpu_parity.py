@@ -285,3 +285,37 @@ def test_heterogeneous_launch(session, oracle_lib, net14, net54, precision):
         we = weighted_err(y, yo)
         print("heterogeneous precision %d, %s-state part: max weighted err %.3g" % (precision, label, we.max()))
         assert we.max() <= 10.0
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_graph_replay_matches_eager(session, net14, precision):
+    """hipGraph of one global step replayed four times, with eager integrations
+    of another buffer between the replays, against the eager step-by-step run:
+    every replay bit-identical."""
+    from hipmem import DeviceArray
+    num, dt = 1024, 1e-6
+    y0, P = M.initial_conditions(net14, num, seed=41)
+    _kin_setup(net14, num)
+    acc.set_precision(precision)
+    try:
+        p = DeviceArray(P)
+        ya = DeviceArray(y0.T)
+        snaps = []
+        for _ in range(4):
+            acc.integrate_device(num, 0.0, dt, dt, ya.addr, p.addr)
+            acc.synchronize()
+            snaps.append(ya.get())
+        yb = DeviceArray(y0.T)
+        other = DeviceArray(y0.T)
+        acc.graph_capture(num, 0.0, dt, dt, yb.addr, p.addr)
+        assert np.array_equal(yb.get(), np.ascontiguousarray(y0.T))  # capture runs nothing
+        for k in range(4):
+            acc.graph_launch(1)
+            acc.synchronize()
+            assert np.array_equal(yb.get(), snaps[k]), k
+            acc.integrate_device(num, 0.0, dt, dt, other.addr, p.addr)  # eager work between replays
+            acc.synchronize()
+        assert np.array_equal(other.get(), snaps[3])
+    finally:
+        acc.graph_release()
+        acc.set_precision(acc.PRECISION_FP64)
