@@ -103,7 +103,7 @@ def lib():
         "accelerInt_mlp_release": ([], None),
         "accelerInt_set_learned_start": ([i], i),
         "accelerInt_log_first_steps": ([v, v, i, v], i),
-        "accelerInt_mlp_predict_states": ([i, d, v, v], i),
+        "accelerInt_mlp_predict_states": ([i, d, v, v, v], i),
         "accelerInt_integrate_device_hetero": ([i, v, v, v, i, v, v, v, d, d, d, v], i),
         "accelerInt_mlp_train_host": ([i, v, v, i, i, ctypes.c_float, ctypes.c_uint, v], i),
         "accelerInt_start_log": ([i], i),
@@ -205,7 +205,7 @@ def mlp_set_normalization(mu, sd, zscale):
 
 
 def mlp_loss_grad(x, t):
-    """(loss, grad) of a normalized batch x[B][N+1], t[B][3N] (B a multiple of 16)."""
+    """(loss, grad) of a normalized batch x[B][N+2], t[B][3N] (B a multiple of 16)."""
     x, t = _f32(x), _f32(t)
     n = _mlp_check(lib().accelerInt_mlp_param_count(), "accelerInt_mlp_param_count")
     g = np.zeros(n, dtype=np.float32)
@@ -230,12 +230,15 @@ def mlp_train(n: int, x_dev_ptr: int, t_dev_ptr: int, epochs: int, batch: int, l
     return hist
 
 
-def mlp_predict_states(y_cond_major, h: float):
-    """The integrator's in-kernel inference on states y[num][N] (54-state networks)."""
+def mlp_predict_states(y_cond_major, h: float, pr):
+    """The integrator's in-kernel inference on states y[num][N] at parameters
+    pr[num] (54-state networks)."""
     y = np.ascontiguousarray(y_cond_major, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    assert pr.shape == (y.shape[0],)
     z = np.zeros((y.shape[0], 3 * y.shape[1]), dtype=np.float64)
-    _mlp_check(lib().accelerInt_mlp_predict_states(y.shape[0], float(h), y.ctypes.data, z.ctypes.data),
-               "accelerInt_mlp_predict_states")
+    _mlp_check(lib().accelerInt_mlp_predict_states(y.shape[0], float(h), y.ctypes.data, pr.ctypes.data,
+                                                   z.ctypes.data), "accelerInt_mlp_predict_states")
     return z
 
 
@@ -253,8 +256,9 @@ def start_log(capacity: int):
 
 
 def read_log(capacity: int, n_states: int):
-    """(X, Z) of the library-owned log: X[i] = (y, log h), Z[i] = (Z1, Z2, Z3)."""
-    x = np.zeros((capacity, n_states + 1), dtype=np.float32)
+    """(X, Z) of the library-owned log: X[i] = features (1000/T, log Y_k, log h,
+    log p), Z[i] = (Z1, Z2, Z3)."""
+    x = np.zeros((capacity, n_states + 2), dtype=np.float32)
     z = np.zeros((capacity, 3 * n_states), dtype=np.float32)
     n = _check(lib().accelerInt_read_log(x.ctypes.data, z.ctypes.data, int(capacity)), "accelerInt_read_log")
     return x[:n], z[:n]

@@ -57,14 +57,19 @@ __device__ __forceinline__ int grab_condition(int* work) {
 }
 
 // Learned Newton start for the first step of a global step (one condition
-// per wavefront; the network's system size must match).
+// per wavefront; the network's system size must match).  A prediction with a
+// non-finite component anywhere in the group is dropped (zero start), and a
+// condition whose predicted start was once discarded (see Radau::StartFailed)
+// gets none for the rest of the launch.
 template <class Mech, int G, int R, class St>
-__device__ __forceinline__ void predict_start(const SolverParams& P, St& st) {
-  if constexpr (G == 64 && R == 1 && Mech::N < 64) {
+__device__ __forceinline__ void predict_start(const SolverParams& P, double pr, St& st) {
+  if constexpr (G == 64 && R == 1 && Mech::N + 2 <= 64) {
     if (P.mlp.on && P.mlp.n == Mech::N) {
       double z1[1], z2[1], z3[1];
-      mlp_predict_wave<Mech::N>(P.mlp, st.y[0], log(st.H), z1[0], z2[0], z3[0]);
-      st.set_start(z1, z2, z3);
+      if (st.StartFailed) return;
+      mlp_predict_wave<Mech::N>(P.mlp, st.y[0], log(st.H), log(pr), z1[0], z2[0], z3[0]);
+      const bool bad = !(__builtin_isfinite(z1[0]) && __builtin_isfinite(z2[0]) && __builtin_isfinite(z3[0]));
+      if (Grp<G>::sum(bad ? 1.0 : 0.0) == 0.0) st.set_start(z1, z2, z3);
     }
   }
 }
@@ -116,10 +121,11 @@ __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, doubl
 #pragma unroll
       for (int i = 0; i < NCNT; ++i) cnt[i] = 0;
       prc = qa.pr[c];
+      st.StartFailed = false;
       k = 0;
       t_next = fmin(t_end, t_start + step);
       st.begin(P, t_start, t_next);
-      predict_start<Mech, G, R>(P, st);
+      predict_start<Mech, G, R>(P, prc, st);
       fresh = false;
     }
     int code = EC_success;
@@ -130,7 +136,7 @@ __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, doubl
       ++k;
       t_next = fmin(t_end, t_start + (k + 1) * step);
       st.begin(P, t0, t_next);
-      predict_start<Mech, G, R>(P, st);
+      predict_start<Mech, G, R>(P, prc, st);
       continue;
     }
     // condition finished (all steps done, or an error code): write back, or
@@ -703,7 +709,7 @@ int accelerInt_start_log(int capacity) {
   if (S.own_log_n) hipFree(S.own_log_n);
   S.own_log_x = S.own_log_z = nullptr;
   S.own_log_n = nullptr;
-  if (!HIPOK(hipMalloc(&S.own_log_x, (size_t)capacity * (S.n + 1) * sizeof(float))) ||
+  if (!HIPOK(hipMalloc(&S.own_log_x, (size_t)capacity * (S.n + 2) * sizeof(float))) ||
       !HIPOK(hipMalloc(&S.own_log_z, (size_t)capacity * 3 * S.n * sizeof(float))) ||
       !HIPOK(hipMalloc(&S.own_log_n, sizeof(int))) || !HIPOK(hipMemset(S.own_log_n, 0, sizeof(int))))
     return -1;
@@ -723,7 +729,7 @@ int accelerInt_read_log(float* x_host, float* z_host, int capacity) {
     return -1;
   if (n > S.own_log_cap) n = S.own_log_cap;
   if (n > capacity) n = capacity;
-  if (n > 0 && ((x_host && !HIPOK(hipMemcpy(x_host, S.own_log_x, (size_t)n * (S.n + 1) * sizeof(float),
+  if (n > 0 && ((x_host && !HIPOK(hipMemcpy(x_host, S.own_log_x, (size_t)n * (S.n + 2) * sizeof(float),
                                              hipMemcpyDeviceToHost))) ||
                 (z_host && !HIPOK(hipMemcpy(z_host, S.own_log_z, (size_t)n * 3 * S.n * sizeof(float),
                                              hipMemcpyDeviceToHost)))))

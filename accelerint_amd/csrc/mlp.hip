@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <cmath>
 #include <random>
 #include <string>
 #include <vector>
@@ -134,16 +135,18 @@ __global__ void k_mlp_gather(int rows, int cols, int ld, const int* __restrict__
 }
 
 // In-integrator inference path on given states: one wavefront per condition,
-// lane r < N holding y_r (condition-major y[c*N + r]); z[c*3N + s*N + r].
+// lane r < N holding y_r (condition-major y[c*N + r]), parameter pr[c];
+// z[c*3N + s*N + r].
 template <int N>
 __global__ __launch_bounds__(64) void k_mlp_predict_states(int num, double logh, const double* __restrict__ y,
-                                                           double* __restrict__ z, MlpView m) {
+                                                           const double* __restrict__ pr, double* __restrict__ z,
+                                                           MlpView m) {
   const int c = blockIdx.x;
   if (c >= num) return;
   const int l = (int)threadIdx.x;
   const double yr = l < N ? y[(size_t)c * N + l] : 0.0;
   double z1, z2, z3;
-  mlp_predict_wave<N>(m, yr, logh, z1, z2, z3);
+  mlp_predict_wave<N>(m, yr, logh, log(pr[c]), z1, z2, z3);
   if (l < N) {
     z[(size_t)c * 3 * N + l] = z1;
     z[(size_t)c * 3 * N + N + l] = z2;
@@ -330,14 +333,14 @@ using namespace acc;
 extern "C" {
 
 int accelerInt_mlp_init(int n_states, unsigned seed) {
-  if (n_states < 1 || n_states + 1 > 64) {
-    G.err = "accelerInt_mlp_init: n_states must be in 1..63";
+  if (n_states < 1 || n_states + 2 > 64) {
+    G.err = "accelerInt_mlp_init: n_states must be in 1..62";
     return -1;
   }
   free_all();
   const int H = kMlpHidden;
   G.n = n_states;
-  G.in = n_states + 1;
+  G.in = n_states + 2;
   G.inp = mlp_pad16(G.in);
   G.out = 3 * n_states;
   G.outp = mlp_pad16(G.out);
@@ -489,29 +492,36 @@ int accelerInt_mlp_train(int n, const float* x_dev, const float* t_dev, int epoc
                          b1, b2, 1e-8f, c1, c2);
     }
     if (loss_per_epoch) loss_per_epoch[ep] = (float)(sse / ((double)steps * batch * G.out));
+    if (rc == 0 && !std::isfinite(sse)) {
+      G.err = "accelerInt_mlp_train: non-finite loss (training diverged or non-finite samples)";
+      rc = -1;
+    }
   }
   hipFree(d_perm);
   if (rc == 0 && !refresh_swizzled()) rc = -1;
   return rc;
 }
 
-int accelerInt_mlp_predict_states(int num, double h, const double* y, double* z) {
-  if (G.n != 54 || num <= 0 || !y || !z || !(h > 0)) {
+int accelerInt_mlp_predict_states(int num, double h, const double* y, const double* pr, double* z) {
+  if (G.n != 54 || num <= 0 || !y || !pr || !z || !(h > 0)) {
     G.err = "accelerInt_mlp_predict_states: needs a 54-state network, num > 0, h > 0";
     return -1;
   }
   const int N = 54;
-  double *dy = nullptr, *dz = nullptr;
+  double *dy = nullptr, *dz = nullptr, *dp = nullptr;
   int rc = -1;
   if (ok(hipMalloc(&dy, (size_t)num * N * 8), "hipMalloc") && ok(hipMalloc(&dz, (size_t)num * 3 * N * 8), "hipMalloc") &&
-      ok(hipMemcpy(dy, y, (size_t)num * N * 8, hipMemcpyHostToDevice), "hipMemcpy")) {
-    hipLaunchKernelGGL(k_mlp_predict_states<54>, dim3(num), dim3(64), 0, 0, num, log(h), dy, dz, mlp_view());
+      ok(hipMalloc(&dp, (size_t)num * 8), "hipMalloc") &&
+      ok(hipMemcpy(dy, y, (size_t)num * N * 8, hipMemcpyHostToDevice), "hipMemcpy") &&
+      ok(hipMemcpy(dp, pr, (size_t)num * 8, hipMemcpyHostToDevice), "hipMemcpy")) {
+    hipLaunchKernelGGL(k_mlp_predict_states<54>, dim3(num), dim3(64), 0, 0, num, log(h), dy, dp, dz, mlp_view());
     if (ok(hipGetLastError(), "k_mlp_predict_states") &&
         ok(hipMemcpy(z, dz, (size_t)num * 3 * N * 8, hipMemcpyDeviceToHost), "hipMemcpy"))
       rc = 0;
   }
   if (dy) hipFree(dy);
   if (dz) hipFree(dz);
+  if (dp) hipFree(dp);
   return rc;
 }
 

@@ -1,9 +1,13 @@
 // Learned Newton start: a small MLP that predicts the Radau stage values Z1..Z3
-// of a global step's first step from (state, log h).  No counterpart in the
-// reference (radau2a.c:600-603 starts every Newton iteration of a first step
-// at zero); the integrator keeps the reference's method as the fallback.
+// of a global step's first step from (state, log h, log p).  No counterpart in
+// the reference (radau2a.c:600-603 starts every Newton iteration of a first
+// step at zero); the integrator keeps the reference's method as the fallback.
 //
-// Network: in = N + 1 features, two hidden layers of 128 (GELU, tanh form),
+// Features (kinetics state layout y = (T, Y_1..Y_ns)): 1000 / T, log of each
+// mass fraction (floored at 1e-30), log h, log p (the condition's parameter,
+// pressure: the right-hand side depends on it, so the stages do too).
+//
+// Network: in = N + 2 features, two hidden layers of 128 (GELU, tanh form),
 // out = 3N; padded to multiples of 16 (IN_P, 128, OUT_P).  Matrix products run
 // on the BF16 MFMA units (v_mfma_f32_16x16x16_bf16) with FP32 accumulation;
 // master weights, biases and normalization stay FP32.
@@ -23,6 +27,17 @@ typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMlpHidden = 128;
+// Normalized input features are clamped to [-kMlpClip, kMlpClip] (the host
+// applies the same clamp to training data): a species that is absent from
+// almost every sample has a tiny spread, and one sample where it is present
+// would otherwise feed a feature of thousands into the network.
+constexpr float kMlpClip = 16.0f;
+
+// Feature of state component r (kinetics layout: r = 0 temperature, else a
+// mass fraction).
+__device__ inline float mlp_feature(int r, double v) {
+  return r == 0 ? (float)(1000.0 / v) : (float)log(fmax(v, 1e-30));
+}
 
 __host__ __device__ constexpr int mlp_pad16(int n) { return (n + 15) / 16 * 16; }
 
@@ -60,7 +75,7 @@ struct MlpView {
   const float* b1;
   const float* b2;
   const float* b3;
-  const float* mu;   // [IN_P] feature means (features: y_0..y_{N-1}, log h)
+  const float* mu;   // [IN_P] feature means (features: mlp_feature(y_r), log h, log p)
   const float* sd;   // [IN_P] feature scales
   const float* zs;   // [OUT_P] output scales: Z_j = zs[j] * out_j
   int n;             // system size N
@@ -71,7 +86,7 @@ struct MlpView {
 // each global step, features (y at the step start, log h) and the converged
 // stages (Z1, Z2, Z3), as float.
 struct StartLog {
-  float* x;      // [cap][N + 1]
+  float* x;      // [cap][N + 2] features (mlp_feature(y_r), log h, log p)
   float* z;      // [cap][3 N]
   int* count;
   int cap;
@@ -81,14 +96,14 @@ struct StartLog {
 
 namespace acc {
 
-// One condition per wavefront (64-lane group, N <= 63): lane r < N holds y_r;
+// One condition per wavefront (64-lane group, N <= 62): lane r < N holds y_r;
 // returns in lane r the predicted stages Z1[r], Z2[r], Z3[r] for a step of
-// size h.  Row 0 of each MFMA tile carries the condition; the other 15 rows
+// size h at parameter p (logp = log p).  Row 0 of each MFMA tile carries the condition; the other 15 rows
 // are zero (the per-wave cost is ~200 MFMA instructions).
 template <int N>
-__device__ inline void mlp_predict_wave(const MlpView& m, double yr, double logh, double& z1, double& z2,
-                                        double& z3) {
-  constexpr int IN = N + 1, INP = mlp_pad16(IN), OUT = 3 * N, OUTP = mlp_pad16(OUT), H = kMlpHidden;
+__device__ inline void mlp_predict_wave(const MlpView& m, double yr, double logh, double logp, double& z1,
+                                        double& z2, double& z3) {
+  constexpr int IN = N + 2, INP = mlp_pad16(IN), OUT = 3 * N, OUTP = mlp_pad16(OUT), H = kMlpHidden;
   constexpr int KT1 = INP / 16, NT1 = H / 16, NT3 = OUTP / 16;
   static_assert(IN <= 64, "one feature per lane");
   const int l = (int)__lane_id();
@@ -96,8 +111,8 @@ __device__ inline void mlp_predict_wave(const MlpView& m, double yr, double logh
   const int kq = 4 * (l >> 4);
   float x = 0.0f;
   if (l < IN) {
-    const float f = (l < N) ? (float)yr : (float)logh;
-    x = (f - m.mu[l]) / m.sd[l];
+    const float f = (l < N) ? mlp_feature(l, yr) : (float)(l == N ? logh : logp);
+    x = fminf(fmaxf((f - m.mu[l]) / m.sd[l], -kMlpClip), kMlpClip);
   }
   // layer 1: features -> 128
   mlp_f32x4 h[NT1];

@@ -900,6 +900,9 @@ struct Radau {
   // (zero start, the reference's rule) after any rejection of that step
   double Zs1[R], Zs2[R], Zs3[R];
   bool UseStart;
+  // set when a predicted start was discarded (by the first-iteration test or
+  // a Newton failure); the caller stops predicting for this condition
+  bool StartFailed;
 
   __device__ __forceinline__ void set_start(const double (&z1)[R], const double (&z2)[R], const double (&z3)[R]) {
 #pragma unroll
@@ -1045,7 +1048,11 @@ struct Radau {
     return info != 0 ? info : info2;
   }
 
-  // RK_ErrorNorm, radau2a.c:446-453: max(sqrt(sum (sc d)^2 / N), 1e-10)
+  // RK_ErrorNorm, radau2a.c:446-453: max(sqrt(sum (sc d)^2 / N), 1e-10).
+  // A NaN norm is returned as +inf: fmax would turn it into 1e-10 and let a
+  // non-finite Newton increment or error estimate pass as converged/accepted
+  // (the reference's fmax does exactly that); +inf makes it a divergence and
+  // a rejection instead.  Finite norms are unchanged.
   __device__ __forceinline__ double err_norm(const double (&d)[R]) const {
     const int ln = Grp<G>::lane();
     double s = 0.0;
@@ -1053,7 +1060,8 @@ struct Radau {
     for (int q = 0; q < R; ++q)
       if (RW::valid(ln, q)) s += sc[q] * sc[q] * d[q] * d[q];
     s = Grp<G>::sum(s);
-    return fmax(sqrt(s / (double)N), 1e-10);
+    const double e = sqrt(s / (double)N);
+    return e == e ? fmax(e, 1e-10) : __builtin_inf();
   }
 
   // integrate() prologue, radau2a.c:521-564: a fresh start for every interval.
@@ -1079,27 +1087,30 @@ struct Radau {
 
   __device__ __forceinline__ bool finished() const { return !(t + kRoundoff < t_end); }
 
-  // Training sample of the learned start: features (y at the step start,
-  // log h) and the converged stages of an accepted first step.
-  __device__ __forceinline__ void log_first_step(const StartLog& L) const {
+  // Training sample of the learned start: features (of y at the step start,
+  // log h, log pr) and the converged stages of an accepted first step.
+  __device__ __forceinline__ void log_first_step(const StartLog& L, double pr) const {
     const int ln = Grp<G>::lane();
     int slot = 0;
     if (ln == 0) slot = atomicAdd(L.count, 1);
     slot = Grp<G>::bcast_i(slot, 0);
     if (slot >= L.cap) return;
-    float* x = L.x + (size_t)slot * (N + 1);
+    float* x = L.x + (size_t)slot * (N + 2);
     float* z = L.z + (size_t)slot * 3 * N;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       if (RW::valid(ln, q)) {
         const int r = RW::row(ln, q);
-        x[r] = (float)y[q];
+        x[r] = mlp_feature(r, y[q]);
         z[r] = (float)Z1[q];
         z[N + r] = (float)Z2[q];
         z[2 * N + r] = (float)Z3[q];
       }
     }
-    if (ln == 0) x[N] = (float)log(H);
+    if (ln == 0) {
+      x[N] = (float)log(H);
+      x[N + 1] = (float)log(pr);
+    }
   }
 
   // One pass of the step loop, radau2a.c:566-734.  Returns RADAU_CONTINUE,
@@ -1207,6 +1218,23 @@ struct Radau {
       const double d2 = err_norm(DZ2);
       const double d3 = err_norm(DZ3);
       const double NewtonIncrement = sqrt((d1 * d1 + d2 * d2 + d3 * d3) / 3.0);
+      if (FirstStep && UseStart && NewtonIter == 0) {
+        // A predicted start is kept only if its first Newton correction is
+        // smaller than the corrected stages, i.e. than the zero start's
+        // distance from the same point; otherwise Newton restarts from zero.
+        double W1[R], W2[R], W3[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) { W1[q] = Z1[q] - DZ1[q]; W2[q] = Z2[q] - DZ2[q]; W3[q] = Z3[q] - DZ3[q]; }
+        const double w1 = err_norm(W1), w2 = err_norm(W2), w3 = err_norm(W3);
+        if (!(NewtonIncrement < sqrt((w1 * w1 + w2 * w2 + w3 * w3) / 3.0))) {
+          UseStart = false;
+          StartFailed = true;
+#pragma unroll
+          for (int q = 0; q < R; ++q) { Z1[q] = 0.0; Z2[q] = 0.0; Z3[q] = 0.0; }
+          NewtonIter = -1;  // the loop's increment makes the restart iteration 0
+          continue;
+        }
+      }
       Theta = kThetaMin;
       if (NewtonIter > 0) {
         Theta = NewtonIncrement / NewtonIncrementOld;
@@ -1232,7 +1260,20 @@ struct Radau {
       // iteration on), never on the rate carried over from the previous step.
       NewtonDone = (!kMixed || NewtonIter > 0) && (NewtonRate * NewtonIncrement <= kNewtonTol);
       if (NewtonDone) break;
-      if (NewtonIter == kNewtonMaxit - 1) return EC_newton_max_iterations_exceeded;
+      if (NewtonIter == kNewtonMaxit - 1) {
+        if (FirstStep && UseStart) break;
+        return EC_newton_max_iterations_exceeded;
+      }
+    }
+    if (!NewtonDone && FirstStep && UseStart) {
+      // A predicted start that does not converge falls back to the zero
+      // start at the same H; the factorizations belong to this H and are kept.
+      cnt[CNT_NEWTON_FAIL]++;
+      UseStart = false;
+      StartFailed = true;
+      SkipJac = true;
+      SkipLU = true;
+      return RADAU_CONTINUE;
     }
     if (P.const_time_step) {  // :725-733
       t += H;
@@ -1281,7 +1322,7 @@ struct Radau {
     double Hnew = Fac * H;
     if (Err < 1.0) {
       cnt[CNT_ACCEPT]++;
-      if (FirstStep && P.log.cap > 0) log_first_step(P.log);
+      if (FirstStep && P.log.cap > 0) log_first_step(P.log, pr);
       if (!FirstStep) {  // Gustafsson, :676-685
         double FacGus = kFacSafe * (H / Hacc) * pow(Err * Err / ErrOld, -0.25);
         FacGus = fmin(kFacMax, fmax(kFacMin, FacGus));

@@ -1,13 +1,15 @@
 """Learned Newton start: sample collection, normalization and on-GPU training.
 
 The integrator logs, for the first accepted step of every global step, the
-state at the step start, log h and the converged Radau stages (Z1, Z2, Z3)
-(``accelerInt_start_log`` / ``accelerInt_read_log``).  A network
-N + 1 -> 128 -> 128 -> 3N is trained on them with the library's BF16-MFMA
-kernels (``accelerInt_mlp_train_host``) and then predicts the Newton start of
-each global step's first step inside the integrator
-(``accelerInt_set_learned_start``); the reference's zero start
-(radau2a.c:600-603) remains the fallback after any rejection of that step.
+features of the step start (1000/T, log of each mass fraction, log h, log p)
+and the converged Radau stages (Z1, Z2, Z3) (``accelerInt_start_log`` /
+``accelerInt_read_log``).  A network N + 2 -> 128 -> 128 -> 3N is trained on
+them with the library's BF16-MFMA kernels (``accelerInt_mlp_train_host``) and
+then predicts the Newton start of each global step's first step inside the
+integrator (``accelerInt_set_learned_start``); the reference's zero start
+(radau2a.c:600-603) takes over when the prediction's first Newton correction
+is not smaller than the corrected stages, when Newton from it fails, and
+after any rejection of that step.
 """
 from __future__ import annotations
 
@@ -19,7 +21,7 @@ from . import (PRECISION_FP64, get_counters, get_status, integrate, mlp_init, ml
 
 def collect(y0, pr, steps: int, dt: float, precision: int = PRECISION_FP64):
     """Integrates ``steps`` global steps of dt from y0 (column-major N x NUM)
-    with sample logging on; returns (X, Z) with X[i] = (y, log h) and
+    with sample logging on; returns (X, Z) with X[i] = the features and
     Z[i] = (Z1, Z2, Z3) as float64 arrays."""
     n, num = y0.shape
     cap = num * steps
@@ -31,7 +33,10 @@ def collect(y0, pr, steps: int, dt: float, precision: int = PRECISION_FP64):
     finally:
         X, Z = read_log(cap, n)
         set_precision(PRECISION_FP64)
-    return X.astype(np.float64), Z.astype(np.float64)
+    # a mixed-precision pass that is later re-run with FP64 factors may have
+    # logged samples before it failed; only finite samples are kept
+    keep = np.isfinite(X).all(1) & np.isfinite(Z).all(1)
+    return X[keep].astype(np.float64), Z[keep].astype(np.float64)
 
 
 def normalization(X, Z):
@@ -44,6 +49,9 @@ def normalization(X, Z):
     return mu, sd, zs
 
 
+CLIP = 16.0  # clamp of normalized features, the same as the kernels' kMlpClip
+
+
 def train(X, Z, epochs: int = 40, batch: int = 512, lr: float = 2e-3, seed: int = 0):
     """Fresh network for N = Z.shape[1] / 3, trained on (X, Z); returns the
     per-epoch mean squared error on normalized targets."""
@@ -52,8 +60,8 @@ def train(X, Z, epochs: int = 40, batch: int = 512, lr: float = 2e-3, seed: int 
     mlp_init(n, seed)
     mlp_set_normalization(mu, sd, zs)
     m = (X.shape[0] // batch) * batch
-    return mlp_train_host(((X[:m] - mu) / sd).astype(np.float32), (Z[:m] / zs).astype(np.float32), epochs, batch,
-                          lr, seed)
+    xn = np.clip((X[:m] - mu) / sd, -CLIP, CLIP).astype(np.float32)
+    return mlp_train_host(xn, (Z[:m] / zs).astype(np.float32), epochs, batch, lr, seed)
 
 
 def integrate_ab(y0, pr, steps: int, dt: float, precision: int, learned: bool):

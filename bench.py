@@ -1,14 +1,17 @@
 """Benchmark: conditions/s per 1e-6 s global step of the batched Radau IIA-5
 integrator on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
 
-Workload (default ``c2``): the 54-state synthetic constant-pressure kinetics
+Workload (default ``c3``): the 54-state synthetic constant-pressure kinetics
 network (53 species + T, 325 reversible reactions), 131072 conditions with
 T ~ U[800, 2200] K, P ~ U[1, 25] atm, Y ~ Dirichlet(0.5), rtol 1e-6,
 atol 1e-10, global steps of 1e-6 s, mixed precision (FP32 iteration-matrix LU
-and Newton solves, FP64 residuals, error estimate, step control and state),
-with the plain FP64 method timed on the same inputs for comparison.  ``c1`` is
+and Newton solves, FP64 residuals, error estimate, step control and state)
+and the learned Newton start (network trained, untimed, on samples the
+integrator logs from the same conditions), with the plain FP64 method and
+mixed precision without the learned start timed on the same post-warmup
+states for comparison.  ``c2`` is the same without the learned start, ``c1``
 the 14-state network (262144 conditions, plain FP64), ``c0`` van der Pol.
 
 A step is one global step of every condition.  W warmup steps run untimed;
@@ -38,7 +41,8 @@ PROFILES = os.path.join(ROOT, "profiles")
 # measured HBM traffic per condition-step of the integration kernel, by workload
 # and precision (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated): the profile
 # summaries under profiles/ carry the numbers and the commands that made them
-TRAFFIC_PROFILES = {("c2", "mixed"): "k_radau_kinetics54_mixed_131072.md"}
+TRAFFIC_PROFILES = {("c2", "mixed"): "k_radau_kinetics54_mixed_131072.md",
+                    ("c3", "mixed"): "k_radau_kinetics54_mixed_131072.md"}
 
 
 def measured_traffic(config, precision):
@@ -57,8 +61,13 @@ CONFIGS = {
            "synthetic kinetics 14 states (13 species + T), 27 reactions, 262144 conditions, dt=1e-6 s"),
     "c2": (54, 131072, 1e-6, (800.0, 2200.0),
            "synthetic kinetics 54 states (53 species + T), 325 reactions, 131072 conditions, dt=1e-6 s"),
+    "c3": (54, 131072, 1e-6, (800.0, 2200.0),
+           "synthetic kinetics 54 states (53 species + T), 325 reactions, 131072 conditions, dt=1e-6 s, "
+           "learned Newton start (MLP 55-128-128-162 GELU, BF16 MFMA) trained on logged (y, log h, Z) samples"),
 }
-DEFAULT_PRECISION = {"c0": "fp64", "c1": "fp64", "c2": "mixed"}
+DEFAULT_PRECISION = {"c0": "fp64", "c1": "fp64", "c2": "mixed", "c3": "mixed"}
+LEARNED = {"c3"}
+HEADLINE = "conditions/sec per 1e-6 s step: mixed+learned vs plain FP64 Radau, same error"
 
 
 def log(*a):
@@ -132,13 +141,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--num", type=int, default=0, help="override condition count")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None,
                     help="iteration-matrix arithmetic (default: mixed for c2, fp64 otherwise)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain-FP64 comparison leg of a mixed-precision run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--train-samples", type=int, default=2000000,
+                    help="learned-start configs: logged samples to train on (first accepted step of each "
+                         "condition and global step, from an untimed run)")
+    ap.add_argument("--train-epochs", type=int, default=3)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (rocprof traffic calibration)")
     args = ap.parse_args()
@@ -180,10 +193,12 @@ def main():
     pr_dev = torch.from_numpy(pr).to(dev)
     codes_dev = torch.zeros(num, dtype=torch.int32, device=dev)
 
-    def run_leg(precision, y_start=None):
+    def run_leg(precision, y_start=None, learned=False):
         """Warmup W global steps (untimed; skipped when the post-warmup states
         are given), then K timed global steps in one persistent launch."""
         acc.set_precision(acc.PRECISION_MIXED if precision == "mixed" else acc.PRECISION_FP64)
+        acc.set_learned_start(learned)
+        label = precision + ("+learned" if learned else "")
         t = args.warmup * dt
         if y_start is None:
             y_dev = torch.from_numpy(np.ascontiguousarray(y0.T)).to(dev)  # condition-major
@@ -211,45 +226,80 @@ def main():
         cnt = acc.get_counters(num)
         flops = algorithmic_flops(cnt, n, F, Jc)
         per_unit = cnt.sum(0) / (num * args.steps)
-        log("[%s] per condition-step events: " % precision
+        log("[%s] per condition-step events: " % label
             + ", ".join("%s=%.2f" % (k, v) for k, v in zip(acc.CNT_NAMES, per_unit)))
         log("[%s] integration launch %.3f ms (%d global steps); wall %.3f s; nonzero codes %d; finite %s; "
             "algorithmic FLOPs per condition-step %.3g" % (
-                precision, ms, args.steps, wall, int((codes != 0).sum()), bool(np.isfinite(yf).all()),
+                label, ms, args.steps, wall, int((codes != 0).sum()), bool(np.isfinite(yf).all()),
                 flops / (num * args.steps)))
         esc = acc.escalated() if precision == "mixed" else 0
         if precision == "mixed":
-            log("[mixed] conditions re-run with FP64 factors: %d" % esc)
+            log("[%s] conditions re-run with FP64 factors: %d" % (label, esc))
+        acc.set_learned_start(False)
         return {"wall": wall, "ms": ms, "codes": codes, "y": yf, "flops": flops, "escalated": esc,
-                "y_warm": y_warm}
+                "y_warm": y_warm, "newton": float(per_unit[acc.CNT_NAMES.index("newton")])}
 
     precision = args.precision or DEFAULT_PRECISION[args.config]
+    learned_on = args.config in LEARNED
     log("config %s: %s" % (args.config, desc))
-    main_leg = run_leg(precision)
+    training = None
+    if learned_on:
+        # untimed: log the first accepted step of every condition and global
+        # step from the initial states, then train the start network on them
+        from accelerint_amd import learned
+        steps_log = max(1, -(-args.train_samples // num))
+        t0 = time.perf_counter()
+        X, Z = learned.collect(y0, pr, steps_log, dt, precision=acc.PRECISION_MIXED)
+        t_col = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        hist = learned.train(X, Z, epochs=args.train_epochs, batch=512, lr=2e-3, seed=1)
+        t_train = time.perf_counter() - t0
+        training = {"samples": int(X.shape[0]), "logged_global_steps": steps_log, "epochs": args.train_epochs,
+                    "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1]),
+                    "collect_s": round(t_col, 2), "train_s": round(t_train, 2)}
+        log("learned start training: %s" % json.dumps(training))
+    main_leg = run_leg(precision, learned=learned_on)
     compare = None
     if precision == "mixed" and not args.no_compare:
-        ref = run_leg("fp64", y_start=main_leg["y_warm"])  # same post-warmup states
+        legs = {"fp64": run_leg("fp64", y_start=main_leg["y_warm"])}  # same post-warmup states
+        if learned_on:
+            legs["mixed"] = run_leg("mixed", y_start=main_leg["y_warm"])
+        ref = legs["fp64"]
         diff = np.abs(main_leg["y"] - ref["y"]) / (1e-10 + 1e-6 * np.abs(ref["y"]))
         cond_diff = diff.max(0)
-        log("mixed vs fp64 per-condition max weighted difference: median %.3g p99 %.3g p99.99 %.3g max %.3g"
+        log("main vs fp64 per-condition max weighted difference: median %.3g p99 %.3g p99.99 %.3g max %.3g"
             % (np.median(cond_diff), np.percentile(cond_diff, 99), np.percentile(cond_diff, 99.99),
                cond_diff.max()))
+        compare = {"plain_fp64_value": num * args.steps / ref["wall"],
+                   "speedup_vs_plain_fp64": ref["wall"] / main_leg["wall"],
+                   "max_weighted_diff_vs_fp64": float(diff.max()),
+                   "frac_conditions_within_10x_of_fp64": float((cond_diff <= 10.0).mean()),
+                   "plain_fp64_nonzero_codes": int((ref["codes"] != 0).sum()),
+                   "main_nonzero_codes": int((main_leg["codes"] != 0).sum()),
+                   "escalated_to_fp64": int(main_leg["escalated"])}
+        if learned_on:
+            m = legs["mixed"]
+            compare.update({"mixed_without_learned_value": num * args.steps / m["wall"],
+                            "learned_speedup_vs_mixed": m["wall"] / main_leg["wall"],
+                            "newton_per_condition_step": {"mixed+learned": main_leg["newton"],
+                                                          "mixed": m["newton"], "fp64": ref["newton"]}})
         if mech is not None and not args.no_cpu:
-            # arbitrate the largest differences with the FP64 CPU oracle from the same states
+            # the conditions that differ most from the FP64 result, against the
+            # FP64 CPU oracle and a tight-tolerance (rtol 1e-12) oracle solution
+            # from the same states: equal error means no worse against the latter
             import oracle
             oracle.build()
             om = oracle.Mechanism.kinetics(mech)
             worst = np.argsort(cond_diff)[-8:]
-            yo, _, _ = oracle.integrate_steps(om, np.ascontiguousarray(main_leg["y_warm"][:, worst]),
-                                              np.ascontiguousarray(pr[worst]), dt, args.steps, nthreads=8)
-            wo = lambda y: float((np.abs(y[:, worst] - yo) / (1e-10 + 1e-6 * np.abs(yo))).max())
-            log("8 largest differences vs the CPU oracle: mixed %.3g, fp64 %.3g" % (wo(main_leg["y"]), wo(ref["y"])))
-        compare = {"plain_fp64_value": num * args.steps / ref["wall"],
-                   "speedup_vs_plain_fp64": ref["wall"] / main_leg["wall"],
-                   "max_weighted_diff_vs_fp64": float(diff.max()),
-                   "plain_fp64_nonzero_codes": int((ref["codes"] != 0).sum()),
-                   "mixed_nonzero_codes": int((main_leg["codes"] != 0).sum()),
-                   "escalated_to_fp64": int(main_leg["escalated"])}
+            ys = np.ascontiguousarray(main_leg["y_warm"][:, worst])
+            ps = np.ascontiguousarray(pr[worst])
+            yo, _, _ = oracle.integrate_steps(om, ys, ps, dt, args.steps, nthreads=8)
+            yt, _, _ = oracle.integrate_steps(om, ys, ps, dt, args.steps, nthreads=8, atol=1e-15, rtol=1e-12)
+            werr = lambda y, yr: float((np.abs(y - yr) / (1e-10 + 1e-6 * np.abs(yr))).max())
+            compare["worst8_vs_tight"] = {"main": werr(main_leg["y"][:, worst], yt), "fp64": werr(ref["y"][:, worst], yt),
+                                          "cpu_oracle": werr(yo, yt)}
+            compare["worst8_vs_cpu_oracle"] = {"main": werr(main_leg["y"][:, worst], yo),
+                                               "fp64": werr(ref["y"][:, worst], yo)}
         log("compare: %s" % json.dumps(compare))
 
     if args.calib_copy_mb:
@@ -270,12 +320,17 @@ def main():
     wall = main_leg["wall"]
     method = ("mixed-precision Radau IIA-5 (FP32 iteration-matrix LU and Newton solves, FP64 residuals, "
               "error estimate, step control, state)" if precision == "mixed" else "plain FP64 Radau IIA-5")
+    if learned_on:
+        method += " with the learned Newton start of each global step's first step (zero-start fallback)"
     config = {"workload": "%s: %s" % (args.config, desc), "method": method,
               "rtol": 1e-6, "atol": 1e-10, "n_states": n, "conditions": num, "dt": dt}
+    if training:
+        config["training"] = training
     if compare:
         config.update(compare)
     out = {
-        "metric": "conditions/sec per 1e-6 s step" if n != 2 else "conditions/sec per 1e-3 s step",
+        "metric": HEADLINE if learned_on else (
+            "conditions/sec per 1e-6 s step" if n != 2 else "conditions/sec per 1e-3 s step"),
         "value": num * args.steps / wall,
         "unit": "conditions*steps/s",
         "n_gpus": 1,

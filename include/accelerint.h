@@ -152,11 +152,15 @@ int accelerInt_eval_callbacks(int NUM, double t, const double *y_host, const dou
 int accelerInt_escalated(void);
 
 /* ---- learned Newton start (no reference counterpart) ------------------------
- * A small network (N + 1 -> 128 -> 128 -> 3N, GELU, BF16 MFMA with FP32
+ * A small network (N + 2 -> 128 -> 128 -> 3N, GELU, BF16 MFMA with FP32
  * accumulation) predicts the stage values Z1..Z3 of the first step of each
- * global step from (y, log h), replacing the reference's zero start
- * (radau2a.c:600-603); after any rejection of that step the zero start is used
- * again, and Radau's own error control judges every step.  Features and
+ * global step from the features (1000/T, log Y_1..log Y_ns (floored at 1e-30),
+ * log h, log p), replacing the reference's zero start (radau2a.c:600-603).
+ * The zero start takes over when the first Newton correction from the
+ * prediction is not smaller than the corrected stages, when Newton from it
+ * fails, and after any rejection of that step; Radau's own error control
+ * judges every step.  A condition whose predicted start was discarded once
+ * gets the zero start for the rest of that integration call.  Features and
  * targets passed to the training calls are normalized by the caller
  * (x = (feature - mu) / sd, t = Z / zscale), the same mu, sd and zscale given
  * to accelerInt_mlp_set_normalization for use inside the integrator.
@@ -164,23 +168,23 @@ int accelerInt_escalated(void);
 int accelerInt_mlp_init(int n_states, unsigned seed);
 int accelerInt_mlp_param_count(void);
 /* Parameters in order W1[in_p][128], b1[128], W2[128][128], b2[128],
- * W3[128][out_p], b3[out_p] with in_p = 16*ceil((N+1)/16), out_p = 16*ceil(3N/16). */
+ * W3[128][out_p], b3[out_p] with in_p = 16*ceil((N+2)/16), out_p = 16*ceil(3N/16). */
 int accelerInt_mlp_get_params(float *out, int count);
 int accelerInt_mlp_set_params(const float *in, int count);
 int accelerInt_mlp_set_normalization(const float *mu, const float *sd, const float *zscale);
 /* Mean squared error over a batch (B a multiple of 16) and its gradient
- * (host arrays: x[B][N+1], t[B][3N], grad[param_count]). */
+ * (host arrays: x[B][N+2], t[B][3N], grad[param_count]). */
 int accelerInt_mlp_loss_grad(int B, const float *x, const float *t, float *grad, float *loss);
-/* Batched forward pass (host arrays x[B][N+1] -> y[B][3N], normalized). */
+/* Batched forward pass (host arrays x[B][N+2] -> y[B][3N], normalized). */
 int accelerInt_mlp_predict(int B, const float *x, float *y);
-/* Adam on device-resident samples x_dev[n][N+1], t_dev[n][3N]; loss_per_epoch
+/* Adam on device-resident samples x_dev[n][N+2], t_dev[n][3N]; loss_per_epoch
  * (may be NULL) receives the mean squared error of each epoch. */
 int accelerInt_mlp_train(int n, const float *x_dev, const float *t_dev, int epochs, int batch, float lr,
                          unsigned seed, float *loss_per_epoch);
 /* The integrator's own inference path (one wavefront per condition) on given
- * states: y[num][54] condition-major, step size h -> z[num][3*54] = (Z1, Z2, Z3)
- * denormalized.  54-state networks only. */
-int accelerInt_mlp_predict_states(int num, double h, const double *y, double *z);
+ * states: y[num][54] condition-major, parameters pr[num], step size h ->
+ * z[num][3*54] = (Z1, Z2, Z3) denormalized.  54-state networks only. */
+int accelerInt_mlp_predict_states(int num, double h, const double *y, const double *pr, double *z);
 /* accelerInt_mlp_train on host arrays (copied to the device for the call). */
 int accelerInt_mlp_train_host(int n, const float *x, const float *t, int epochs, int batch, float lr,
                               unsigned seed, float *loss_per_epoch);
@@ -190,8 +194,8 @@ void accelerInt_mlp_release(void);
  * global step's first step (54-state kinetics: one condition per wavefront). */
 int accelerInt_set_learned_start(int on);
 /* Training-data log: while capacity > 0, every integration appends one sample
- * per accepted first step of a global step: x_dev[i][N+1] = (y at the step
- * start, log h), z_dev[i][3N] = converged (Z1, Z2, Z3), as float; *count_dev
+ * per accepted first step of a global step: x_dev[i][N+2] = features of the
+ * step start (as above: 1000/T, log Y_k, log h, log p), z_dev[i][3N] = converged (Z1, Z2, Z3), as float; *count_dev
  * (device int, zeroed by the caller) counts appended samples (it may exceed
  * capacity; samples beyond it are dropped).  capacity 0 turns logging off. */
 int accelerInt_log_first_steps(float *x_dev, float *z_dev, int capacity, int *count_dev);

@@ -35,7 +35,7 @@ def gelu_grad(x):
 class Net:
     def __init__(self, n_states):
         self.n = n_states
-        self.inn, self.out = n_states + 1, 3 * n_states
+        self.inn, self.out = n_states + 2, 3 * n_states
         self.inp, self.outp = pad16(self.inn), pad16(self.out)
         self.shapes = [(self.inp, H), (H,), (H, H), (H,), (H, self.outp), (self.outp,)]
         self.count = sum(int(np.prod(s)) for s in self.shapes)

