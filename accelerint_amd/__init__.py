@@ -27,6 +27,8 @@ MECH_VAN_DER_POL = 0
 MECH_KINETICS = 2
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+SCHEDULE_INDEX = 0
+SCHEDULE_LONGEST_FIRST = 1
 NCNT = 10
 CNT_NAMES = ("step", "accept", "reject", "newton_fail", "lu", "lu_fail", "newton", "f", "jac", "es")
 
@@ -46,6 +48,7 @@ EXPORTS = (
     "accelerInt_set_learned_start", "accelerInt_log_first_steps", "accelerInt_mlp_predict_states",
     "accelerInt_integrate_device_hetero", "accelerInt_mlp_train_host", "accelerInt_start_log",
     "accelerInt_read_log", "accelerInt_graph_capture", "accelerInt_graph_launch", "accelerInt_graph_release",
+    "accelerInt_set_schedule", "accelerInt_get_schedule",
 )
 
 
@@ -91,6 +94,8 @@ def lib():
         "accelerInt_eval_callbacks": ([i, d, v, v, v, v], i),
         "accelerInt_set_precision": ([i], i),
         "accelerInt_escalated": ([], i),
+        "accelerInt_set_schedule": ([i], i),
+        "accelerInt_get_schedule": ([v, v, i], i),
         "accelerInt_mlp_init": ([i, ctypes.c_uint], i),
         "accelerInt_mlp_param_count": ([], i),
         "accelerInt_mlp_get_params": ([v, i], i),
@@ -164,6 +169,24 @@ def set_precision(mode: int):
     """PRECISION_FP64 (default) or PRECISION_MIXED (FP32 iteration-matrix LU and
     Newton solves; FP64 residuals, error estimate, step control and state)."""
     _check(lib().accelerInt_set_precision(int(mode)), "accelerInt_set_precision")
+
+
+def set_schedule(mode: int):
+    """SCHEDULE_LONGEST_FIRST (default): the persistent kernel takes conditions
+    in decreasing order of the work each needed in the previous integration of
+    the same batch size; SCHEDULE_INDEX: index order.  Results are the same."""
+    _check(lib().accelerInt_set_schedule(int(mode)), "accelerInt_set_schedule")
+
+
+def get_schedule(num: int):
+    """(work, order): per-condition work of the last integration (units of one
+    dydt call) and the order the next integration of num conditions will use
+    (None when it will use index order)."""
+    work = np.zeros(num, dtype=np.float32)
+    order = np.zeros(num, dtype=np.int32)
+    k = _check(lib().accelerInt_get_schedule(work.ctypes.data, order.ctypes.data, int(num)),
+               "accelerInt_get_schedule")
+    return work, (order if k == num and num > 0 else None)
 
 
 def escalated() -> int:

@@ -10,6 +10,7 @@
 // global memory, so required_solver_size() is the few bytes of per-condition
 // HBM state.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <math.h>
 #include <stdarg.h>
@@ -56,6 +57,21 @@ __device__ __forceinline__ int grab_condition(int* work) {
   return Grp<G>::bcast_i(c, 0);
 }
 
+// The network's inference as a real call: its temporaries (MFMA tiles,
+// activations) then never compete with the integrator's long-lived registers
+// for the hot loop's allocation; the call's register save/restore happens
+// once per global step.
+struct MlpStages {
+  double z1, z2, z3;
+};
+template <int N>
+__device__ __attribute__((noinline)) MlpStages mlp_predict_call(const MlpView& m, double yr, double logh,
+                                                                double logp) {
+  MlpStages r;
+  mlp_predict_wave<N>(m, yr, logh, logp, r.z1, r.z2, r.z3);
+  return r;
+}
+
 // Learned Newton start for the first step of a global step (one condition
 // per wavefront; the network's system size must match).  A prediction with a
 // non-finite component anywhere in the group is dropped (zero start), and a
@@ -67,7 +83,10 @@ __device__ __forceinline__ void predict_start(const SolverParams& P, double pr, 
     if (P.mlp.on && P.mlp.n == Mech::N) {
       double z1[1], z2[1], z3[1];
       if (st.StartFailed) return;
-      mlp_predict_wave<Mech::N>(P.mlp, st.y[0], log(st.H), log(pr), z1[0], z2[0], z3[0]);
+      const MlpStages zs = mlp_predict_call<Mech::N>(P.mlp, st.y[0], log(st.H), log(pr));
+      z1[0] = zs.z1;
+      z2[0] = zs.z2;
+      z3[0] = zs.z3;
       const bool bad = !(__builtin_isfinite(z1[0]) && __builtin_isfinite(z2[0]) && __builtin_isfinite(z3[0]));
       if (Grp<G>::sum(bad ? 1.0 : 0.0) == 0.0) st.set_start(z1, z2, z3);
     }
@@ -91,12 +110,23 @@ struct Queue {
   const int* list;
   const int* num_dev;
   int* esc;
+  float* cost;  // per-condition work of this call (queue ordering of the next)
 };
+
+// Work of one condition over a call, in units of one right-hand-side
+// evaluation: the relative costs of the events on the 54-state network
+// (LU pair ~12 dydt, Jacobian ~8, a Newton or error-estimate solve ~1).
+__device__ __forceinline__ float work_estimate(const uint32_t (&cnt)[NCNT]) {
+  return (float)cnt[CNT_F] + 12.0f * (float)cnt[CNT_LU] + 8.0f * (float)cnt[CNT_JAC] +
+         (float)(cnt[CNT_NEWTON] + cnt[CNT_ES]);
+}
 
 // The persistent loop of a lane group over a queue: pull a condition, run all
 // of its global steps back to back (one step attempt per iteration), write it
-// back, pull the next.
-template <class Mech, int G, int R, class MT>
+// back, pull the next.  LS: the learned Newton start is compiled in (a
+// separate instantiation, so the network's inference code adds no register
+// pressure to runs without it).
+template <class Mech, int G, int R, class MT, bool LS = false>
 __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, double t_end, double step,
                                           const SolverParams& P, typename Mech::Ctx& ctx) {
   constexpr int N = Mech::N;
@@ -125,7 +155,7 @@ __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, doubl
       k = 0;
       t_next = fmin(t_end, t_start + step);
       st.begin(P, t_start, t_next);
-      predict_start<Mech, G, R>(P, prc, st);
+      if constexpr (LS) predict_start<Mech, G, R>(P, prc, st);
       fresh = false;
     }
     int code = EC_success;
@@ -136,7 +166,7 @@ __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, doubl
       ++k;
       t_next = fmin(t_end, t_start + (k + 1) * step);
       st.begin(P, t0, t_next);
-      predict_start<Mech, G, R>(P, prc, st);
+      if constexpr (LS) predict_start<Mech, G, R>(P, prc, st);
       continue;
     }
     // condition finished (all steps done, or an error code): write back, or
@@ -163,20 +193,21 @@ __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, doubl
 #pragma unroll
         for (int i = 0; i < NCNT; ++i) qa.counters[(size_t)c * NCNT + i] += cnt[i];
       }
+      if (qa.cost) qa.cost[c] = work_estimate(cnt);
     }
     ci = grab_condition<G>(qa.work);
     fresh = true;
   }
 }
 
-template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
+template <class Mech, int G, int R, int BLOCK, int WPE, class MT, bool LS = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
     Queue qa, double t_start, double t_end, double step, SolverParams P, typename Mech::Params mp) {
   extern __shared__ double lds_dyn[];
   const int g = threadIdx.x / G;
   Mech::block_init(mp, lds_dyn);
   typename Mech::Ctx ctx(mp, g, lds_dyn, Radau<Mech, G, R, MT>::kNeedsScratch);
-  run_queue<Mech, G, R, MT>(qa, t_start, t_end, step, P, ctx);
+  run_queue<Mech, G, R, MT, LS>(qa, t_start, t_end, step, P, ctx);
 }
 
 // Heterogeneous batch in one persistent launch: a 54-state and a 14-state
@@ -204,6 +235,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
                             scratch);
     run_queue<MechKinetics14, 16, 1, MT>(q14, t_start, t_end, step, P, ctx);
   }
+}
+
+__global__ void k_iota(int* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i;
 }
 
 __global__ void k_reset_work(int* work, int* esc) {
@@ -304,13 +340,13 @@ std::string g_launch_msg;
 // Persistent grid: as many blocks as fit on the device at once (capped by the
 // number of conditions).  The group's LDS share must fit the device's
 // per-block limit and leave at least one resident block per CU.
-template <class Mech, int G, int R, int BLOCK, int WPE, class MT>
+template <class Mech, int G, int R, int BLOCK, int WPE, class MT, bool LS = false>
 hipError_t launch_radau(int num, double t_start, double t_end, double step, const double* pr,
                         double* y, int* codes, uint32_t* cnt, int* work, const int* list,
                         const int* num_dev, int* esc, const SolverParams& P,
-                        const typename Mech::Params& mp, hipStream_t s) {
+                        const typename Mech::Params& mp, hipStream_t s, float* cost = nullptr) {
   constexpr int GPB = BLOCK / G;
-  auto kern = k_radau<Mech, G, R, BLOCK, WPE, MT>;
+  auto kern = k_radau<Mech, G, R, BLOCK, WPE, MT, LS>;
   const size_t lds = Mech::lds_bytes(mp, GPB, Radau<Mech, G, R, MT>::kNeedsScratch);
   static int resident = 0;     // blocks per device that fit at once, per instantiation
   static size_t lds_ok = 0;    // LDS size the residency was computed for
@@ -342,7 +378,7 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
   int blocks = (num + GPB - 1) / GPB;
   if (blocks > resident) blocks = resident;
   hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work, esc);
-  const Queue qa{num, pr, y, codes, cnt, work, list, num_dev, esc};
+  const Queue qa{num, pr, y, codes, cnt, work, list, num_dev, esc, cost};
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, qa, t_start, t_end, step, P, mp);
   return hipGetLastError();
 }
@@ -379,6 +415,18 @@ struct State {
   int* d_work2 = nullptr;   // second queue of a heterogeneous launch
   int* d_esc2 = nullptr;    // escalation list of the second queue
   int* d_esc = nullptr;     // escalation list of the mixed-precision pass: [count, indices...]
+  // queue order (longest first by the previous call's measured work)
+  int schedule = ACCELERINT_SCHEDULE_LONGEST_FIRST;
+  float* d_cost = nullptr;     // per-condition work of the last call
+  uint32_t* d_keys = nullptr;  // sorted keys (radix-sort output, unused)
+  int* d_iota = nullptr;
+  int* d_order = nullptr;      // condition indices, most work first
+  void* d_sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  int cost_num = 0;   // conditions d_cost describes (0: none)
+  int order_num = 0;  // conditions d_order is a permutation of (0: none)
+  bool capturing = false;
+  bool chunked = false;
   std::vector<int> h_codes;
   int have_status = 0;
   // kinetics tables (device copies)
@@ -428,6 +476,17 @@ void free_buffers() {
   if (S.d_esc) hipFree(S.d_esc);
   if (S.d_work2) hipFree(S.d_work2);
   if (S.d_esc2) hipFree(S.d_esc2);
+  if (S.d_cost) hipFree(S.d_cost);
+  if (S.d_keys) hipFree(S.d_keys);
+  if (S.d_iota) hipFree(S.d_iota);
+  if (S.d_order) hipFree(S.d_order);
+  if (S.d_sort_tmp) hipFree(S.d_sort_tmp);
+  S.d_cost = nullptr;
+  S.d_keys = nullptr;
+  S.d_iota = S.d_order = nullptr;
+  S.d_sort_tmp = nullptr;
+  S.sort_tmp_bytes = 0;
+  S.cost_num = S.order_num = 0;
   S.d_work2 = nullptr;
   S.d_esc2 = nullptr;
   S.d_work = nullptr;
@@ -450,6 +509,18 @@ bool alloc_buffers(int cap) {
   if (!HIPOK(hipMalloc(&S.d_esc, ((size_t)cap + 1) * sizeof(int)))) return false;
   if (!HIPOK(hipMalloc(&S.d_work2, sizeof(int)))) return false;
   if (!HIPOK(hipMalloc(&S.d_esc2, ((size_t)cap + 1) * sizeof(int)))) return false;
+  if (cap > 0) {
+    if (!HIPOK(hipMalloc(&S.d_cost, (size_t)cap * sizeof(float)))) return false;
+    if (!HIPOK(hipMalloc(&S.d_keys, (size_t)cap * sizeof(uint32_t)))) return false;
+    if (!HIPOK(hipMalloc(&S.d_iota, (size_t)cap * sizeof(int)))) return false;
+    if (!HIPOK(hipMalloc(&S.d_order, (size_t)cap * sizeof(int)))) return false;
+    hipLaunchKernelGGL(k_iota, dim3((cap + 255) / 256), dim3(256), 0, 0, S.d_iota, cap);
+    if (!HIPOK(hipGetLastError())) return false;
+    if (!HIPOK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, S.sort_tmp_bytes, (const uint32_t*)S.d_cost,
+                                                             S.d_keys, S.d_iota, S.d_order, cap)))
+      return false;
+    if (!HIPOK(hipMalloc(&S.d_sort_tmp, S.sort_tmp_bytes > 0 ? S.sort_tmp_bytes : 1))) return false;
+  }
   S.padded = cap;
   S.h_codes.assign(cap, 0);
   return true;
@@ -461,23 +532,46 @@ bool alloc_buffers(int cap) {
 template <class MT>
 hipError_t launch_mt(int num, double t_start, double t_end, double step, const double* pr, double* y,
                      int* codes, uint32_t* cnt, const int* list, const int* num_dev, int* esc,
-                     hipStream_t s) {
+                     hipStream_t s, float* cost = nullptr) {
   switch (S.mech) {
     case ACCELERINT_MECH_VAN_DER_POL:
       return launch_radau<MechVdP, 1, 2, 256, ACC_VDP_WPE, MT>(num, t_start, t_end, step, pr, y, codes, cnt,
                                                                S.d_work, list, num_dev, esc, S.P,
-                                                               MechVdP::Params{}, s);
+                                                               MechVdP::Params{}, s, cost);
     case ACCELERINT_MECH_KINETICS:
       if (S.n == 14)
         return launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE, MT>(
-            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp14, s);
+            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp14, s, cost);
+      if (S.n == 54 && S.learned)
+        return launch_radau<MechKinetics54, 64, 1, 256, ACC_K54_WPE, MT, true>(
+            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp54, s, cost);
       if (S.n == 54)
         return launch_radau<MechKinetics54, 64, 1, 256, ACC_K54_WPE, MT>(
-            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp54, s);
+            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp54, s, cost);
       return hipErrorInvalidValue;
     default:
       return hipErrorInvalidValue;
   }
+}
+
+// Queue order of the next launch over num conditions: with the longest-first
+// schedule and the previous call's per-condition work at hand (same number of
+// conditions, one chunk), the condition indices sorted by that work,
+// descending -- greedy list scheduling of the persistent groups then starts
+// the expensive conditions first instead of meeting one at the end.  Inside a
+// graph capture the order is not re-sorted (the sort's own launches stay out
+// of the graph); a captured launch uses the order present at capture.
+// nullptr = index order.  Results do not depend on the order.
+const int* queue_order(int num, hipStream_t s) {
+  if (S.schedule != ACCELERINT_SCHEDULE_LONGEST_FIRST || S.chunked || num < 2 || !S.d_order) return nullptr;
+  if (!S.capturing && S.cost_num == num) {
+    size_t bytes = S.sort_tmp_bytes;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(S.d_sort_tmp, bytes, (const uint32_t*)S.d_cost, S.d_keys,
+                                                     S.d_iota, S.d_order, num, 0, 32, s) != hipSuccess)
+      return nullptr;
+    S.order_num = num;
+  }
+  return S.order_num == num ? S.d_order : nullptr;
 }
 
 // One integration of num <= S.padded conditions.  Mixed precision runs the
@@ -494,11 +588,24 @@ hipError_t launch(int num, double t_start, double t_end, double step, const doub
       return hipErrorInvalidValue;
     }
   }
-  if (S.precision != ACCELERINT_PRECISION_MIXED)
-    return launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, nullptr, s);
-  hipError_t e = launch_mt<float>(num, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, S.d_esc, s);
-  if (e != hipSuccess) return e;
-  return launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, S.d_esc + 1, S.d_esc, nullptr, s);
+  if (num <= 0) {  // occupancy queries only
+    hipError_t e = launch_mt<double>(0, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, nullptr, s);
+    if (e == hipSuccess && S.precision == ACCELERINT_PRECISION_MIXED)
+      e = launch_mt<float>(0, t_start, t_end, step, pr, y, codes, cnt, nullptr, nullptr, nullptr, s);
+    return e;
+  }
+  const int* order = queue_order(num, s);
+  float* cost = S.chunked ? nullptr : S.d_cost;
+  hipError_t e;
+  if (S.precision != ACCELERINT_PRECISION_MIXED) {
+    e = launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, order, nullptr, nullptr, s, cost);
+  } else {
+    e = launch_mt<float>(num, t_start, t_end, step, pr, y, codes, cnt, order, nullptr, S.d_esc, s, cost);
+    if (e == hipSuccess)
+      e = launch_mt<double>(num, t_start, t_end, step, pr, y, codes, cnt, S.d_esc + 1, S.d_esc, nullptr, s, cost);
+  }
+  if (e == hipSuccess && cost) S.cost_num = num;
+  return e;
 }
 
 hipError_t launch_eval_any(int num, double t, const double* pr, const double* y, double* dy,
@@ -587,7 +694,10 @@ int run_steps(int num, double t_start, double t_end, double stepsize, double* y,
   const double step = stepsize < 0 ? t_end - t_start : stepsize;
   if (!(t_start + kRoundoff < t_end) || num <= 0) return 0;
   hipError_t e = hipSuccess;
-  // chunks of at most `padded` conditions (the escalation list's capacity)
+  // chunks of at most `padded` conditions (the escalation list's capacity);
+  // the queue ordering needs the whole batch in one chunk
+  S.chunked = num > S.padded;
+  if (S.chunked) S.cost_num = S.order_num = 0;
   for (int off = 0; off < num && e == hipSuccess; off += S.padded) {
     const int m = num - off < S.padded ? num - off : S.padded;
     e = launch(m, t_start, t_end, step, pr + off, y + (size_t)off * S.n, codes ? codes + off : nullptr,
@@ -680,6 +790,38 @@ int accelerInt_set_precision(int mode) {
   }
   S.precision = mode;
   return 0;
+}
+
+int accelerInt_set_schedule(int mode) {
+  if (mode != ACCELERINT_SCHEDULE_INDEX && mode != ACCELERINT_SCHEDULE_LONGEST_FIRST) {
+    set_err("unknown schedule mode %d", mode);
+    return -1;
+  }
+  S.schedule = mode;
+  return 0;
+}
+
+int accelerInt_get_schedule(float* work, int* order, int num) {
+  if (S.device < 0) return -1;
+  if (num < 0 || num > S.padded) {
+    set_err("accelerInt_get_schedule: num must be 0..%d", S.padded);
+    return -1;
+  }
+  if (!HIPOK(hipStreamSynchronize(S.stream))) return -1;
+  const bool ready = S.schedule == ACCELERINT_SCHEDULE_LONGEST_FIRST && S.cost_num == num && num >= 2;
+  if (ready) {  // the order the next call sorts to
+    size_t bytes = S.sort_tmp_bytes;
+    if (!HIPOK(hipcub::DeviceRadixSort::SortPairsDescending(S.d_sort_tmp, bytes, (const uint32_t*)S.d_cost,
+                                                             S.d_keys, S.d_iota, S.d_order, num, 0, 32, S.stream)) ||
+        !HIPOK(hipStreamSynchronize(S.stream)))
+      return -1;
+    S.order_num = num;
+  }
+  if (work && num > 0 && !HIPOK(hipMemcpy(work, S.d_cost, (size_t)num * sizeof(float), hipMemcpyDeviceToHost)))
+    return -1;
+  if (order && ready && !HIPOK(hipMemcpy(order, S.d_order, (size_t)num * sizeof(int), hipMemcpyDeviceToHost)))
+    return -1;
+  return ready ? num : 0;
 }
 
 int accelerInt_set_learned_start(int on) {
@@ -981,7 +1123,9 @@ int accelerInt_graph_capture(int NUM, double t_start, double t_end, double steps
   if (!hip_ok(launch(0, t_start, t_end, stepsize, var_dev, y_dev, codes_dev, nullptr, s), "occupancy query"))
     return -1;
   if (!HIPOK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed))) return -1;
+  S.capturing = true;
   const int rc = run_steps(NUM, t_start, t_end, stepsize, y_dev, var_dev, codes_dev, nullptr, s);
+  S.capturing = false;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(s, &g);
   if (rc != 0 || !HIPOK(e)) {

@@ -98,6 +98,20 @@ int accelerInt_set_const_time_step(int on);
 #define ACCELERINT_PRECISION_MIXED 1
 int accelerInt_set_precision(int mode);
 
+/* Order in which the persistent kernel takes conditions (no reference
+ * counterpart; results do not depend on it).  LONGEST_FIRST (default): when
+ * the previous integration covered the same number of conditions in one
+ * batch, they are taken in decreasing order of the work each needed then
+ * (greedy list scheduling: the costly conditions start first instead of
+ * finishing last); otherwise, and with INDEX, in index order. */
+#define ACCELERINT_SCHEDULE_INDEX 0
+#define ACCELERINT_SCHEDULE_LONGEST_FIRST 1
+int accelerInt_set_schedule(int mode);
+/* The per-condition work of the last integration (units of one dydt call) and
+ * the order the next one will use (may be NULL).  Returns the number of
+ * conditions the order covers (0: index order), or -1 before initialize. */
+int accelerInt_get_schedule(float *work, int *order, int num);
+
 /* 1 (default): any non-zero condition code aborts the process through
  * check_error, as the reference does.  0: codes are kept and returned by
  * accelerInt_get_status; integration continues for the other conditions. */
@@ -129,7 +143,8 @@ int accelerInt_integrate_device(int NUM, double t_start, double t_end, double st
  * conditions of the installed 14-state network (y14_dev[c*14 + i]) and of the
  * installed 54-state network (y54_dev[c*54 + i]) integrated together, long jobs
  * first; each batch at most NUM-at-initialize conditions; codes arrays may be
- * NULL.  Precision, tolerances and the learned start (54-state part) as set.
+ * NULL.  Precision and tolerances as set; both parts use the zero Newton
+ * start (the learned start is compiled into the single-network kernels only).
  * Asynchronous on `stream`.  Returns 0 or a negative error. */
 int accelerInt_integrate_device_hetero(int NUM14, double *y14_dev, const double *var14_dev, int *codes14_dev,
                                        int NUM54, double *y54_dev, const double *var54_dev, int *codes54_dev,

@@ -323,3 +323,37 @@ def test_graph_replay_matches_eager(session, net14, precision):
     finally:
         acc.graph_release()
         acc.set_precision(acc.PRECISION_FP64)
+
+
+@pytest.mark.parametrize("n,precision", [(14, 0), (54, 1)])
+def test_longest_first_schedule(session, net14, net54, n, precision):
+    """The longest-first queue order: a permutation sorted by the previous
+    call's per-condition work, and results bit-identical to index order."""
+    mech = net14 if n == 14 else net54
+    num, dt = 1024, 1e-6
+    y0, P = M.initial_conditions(mech, num, seed=41, T_range=(800.0, 2200.0))
+    _kin_setup(mech, num)
+    acc.set_precision(precision)
+    out = {}
+    try:
+        for mode in (acc.SCHEDULE_INDEX, acc.SCHEDULE_LONGEST_FIRST):
+            acc.set_schedule(mode)
+            acc.reset_counters()
+            y = y0.copy()
+            acc.integrate(num, 0.0, dt, dt, y, P)
+            work, order = acc.get_schedule(num)
+            if mode == acc.SCHEDULE_INDEX:
+                assert order is None
+            else:
+                assert np.array_equal(np.sort(order), np.arange(num))
+                assert (np.diff(work[order]) <= 0).all()
+                print("work per condition: median %.3g max %.3g" % (np.median(work), work.max()))
+            acc.integrate(num, dt, 4 * dt, dt, y, P)
+            out[mode] = (y, acc.get_status(num), acc.get_counters(num))
+    finally:
+        acc.set_schedule(acc.SCHEDULE_LONGEST_FIRST)
+        acc.set_precision(acc.PRECISION_FP64)
+    a, b = out[acc.SCHEDULE_INDEX], out[acc.SCHEDULE_LONGEST_FIRST]
+    assert np.array_equal(a[0], b[0])
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2], b[2])
