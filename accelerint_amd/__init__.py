@@ -48,7 +48,8 @@ EXPORTS = (
     "accelerInt_set_learned_start", "accelerInt_log_first_steps", "accelerInt_mlp_predict_states",
     "accelerInt_integrate_device_hetero", "accelerInt_mlp_train_host", "accelerInt_start_log",
     "accelerInt_read_log", "accelerInt_graph_capture", "accelerInt_graph_launch", "accelerInt_graph_release",
-    "accelerInt_set_schedule", "accelerInt_get_schedule",
+    "accelerInt_set_schedule", "accelerInt_get_schedule", "read_initial_conditions",
+    "accelerInt_read_initial_conditions", "accelerInt_write_log", "accelerInt_load_kinetics",
 )
 
 
@@ -95,6 +96,9 @@ def lib():
         "accelerInt_set_precision": ([i], i),
         "accelerInt_escalated": ([], i),
         "accelerInt_set_schedule": ([i], i),
+        "accelerInt_read_initial_conditions": ([ctypes.c_char_p, i, v, v], i),
+        "accelerInt_write_log": ([ctypes.c_char_p, i, i, d, v], i),
+        "accelerInt_load_kinetics": ([ctypes.c_char_p], i),
         "accelerInt_get_schedule": ([v, v, i], i),
         "accelerInt_mlp_init": ([i, ctypes.c_uint], i),
         "accelerInt_mlp_param_count": ([], i),
@@ -155,6 +159,29 @@ def set_kinetics(mech) -> None:
     EaR = np.ascontiguousarray(mech.EaR, dtype=np.float64)
     _check(lib().accelerInt_set_kinetics(int(mech.ns), int(mech.nr), _dp(W), _dp(th), _ip(sl),
                                          _dp(lnA), _dp(beta), _dp(EaR)), "accelerInt_set_kinetics")
+
+
+def read_initial_conditions(path: str, num: int):
+    """(y, var) from a file of the reference's initial-condition format
+    (records [t, y_0, param, y_1 .. y_{N-1}]): y column-major (N, num)."""
+    n = num_states()
+    y = np.zeros((n, num), dtype=np.float64)
+    var = np.zeros(num, dtype=np.float64)
+    _check(lib().accelerInt_read_initial_conditions(os.fsencode(path), int(num), y.ctypes.data, var.ctypes.data),
+           "accelerInt_read_initial_conditions")
+    return y, var
+
+
+def write_log(path: str, t: float, y, append: bool = True):
+    """One record of the reference's state log: t, then each condition's N values."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    _check(lib().accelerInt_write_log(os.fsencode(path), int(bool(append)), int(y.shape[1]), float(t),
+                                      y.ctypes.data), "accelerInt_write_log")
+
+
+def load_kinetics(path: str):
+    """Install a kinetics network from a binary network file (``mechanism.write_binary``)."""
+    _check(lib().accelerInt_load_kinetics(os.fsencode(path)), "accelerInt_load_kinetics")
 
 
 def set_tolerances(atol: float, rtol: float):

@@ -1,6 +1,6 @@
 """Build libaccelerint_hip.so in-tree with hipcc for gfx950.
 
-    python -m accelerint_amd.build          # compile (always rebuilds)
+    python -m accelerint_amd.build          # compile the library and the CLI (always rebuilds)
 
 The library is the product: hand-written HIP kernels plus the accelerInt C ABI
 (include/accelerint.h).  No torch types cross the boundary.
@@ -53,6 +53,25 @@ def build(verbose: bool = False, extra_flags=()) -> str:
     return LIB
 
 
+CLI_SRC = os.path.join(HERE, "cli", "accelerint_main.cpp")
+CLI = os.path.join(HERE, "accelerint_main")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The command-line driver (host C++ over the library's C ABI)."""
+    tmp = CLI + ".tmp%d" % os.getpid()
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), CLI_SRC,
+           "-o", tmp, "-L", HERE, "-laccelerint_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building %s" % CLI)
+    os.replace(tmp, CLI)
+    if verbose:
+        print("built %s" % CLI)
+    return CLI
+
+
 def is_stale() -> bool:
     if not os.path.exists(LIB):
         return True
@@ -62,3 +81,4 @@ def is_stale() -> bool:
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_cli(verbose=True)

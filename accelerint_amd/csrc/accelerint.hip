@@ -773,6 +773,108 @@ int accelerInt_set_kinetics(int ns, int nr, const double* W, const double* therm
   return 0;
 }
 
+// ---- files of the reference's command-line driver ---------------------------
+// Initial conditions, generic/read_initial_conditions.c:30-78 with CONP: a
+// binary file of NUM records of N + 2 doubles, [t, y_0, param, y_1 .. y_{N-1}]
+// (y_0 = temperature, param = pressure for kinetics; the state needs no
+// species mask: every species is part of it).
+int accelerInt_read_initial_conditions(const char* filename, int NUM, double* y_host, double* var_host) {
+  const int n = S.n;
+  if (!filename || NUM <= 0 || !y_host || !var_host) {
+    set_err("accelerInt_read_initial_conditions: bad arguments");
+    return -1;
+  }
+  FILE* fp = fopen(filename, "rb");
+  if (!fp) {
+    set_err("Could not open file: %s", filename);
+    return -1;
+  }
+  std::vector<double> rec((size_t)n + 2);
+  for (int i = 0; i < NUM; ++i) {
+    const size_t got = fread(rec.data(), sizeof(double), rec.size(), fp);
+    if (got != rec.size()) {
+      fclose(fp);
+      set_err("File (%s) is incorrectly formatted: record %d has %zu of %d doubles", filename, i, got, n + 2);
+      return -1;
+    }
+    y_host[i] = rec[1];
+    var_host[i] = rec[2];
+    for (int j = 1; j < n; ++j) y_host[i + (size_t)j * NUM] = rec[(size_t)j + 2];
+  }
+  fclose(fp);
+  return 0;
+}
+
+// The reference's own entry point (generic/read_initial_conditions.h:12): it
+// allocates the arrays and exits on a missing or short file.
+void read_initial_conditions(const char* filename, int NUM, double** y_host, double** variable_host) {
+  *y_host = (double*)malloc((size_t)NUM * S.n * sizeof(double));
+  *variable_host = (double*)malloc((size_t)NUM * sizeof(double));
+  if (!*y_host || !*variable_host ||
+      accelerInt_read_initial_conditions(filename, NUM, *y_host, *variable_host) != 0) {
+    fprintf(stderr, "%s\n", S.err.c_str());
+    exit(-1);
+  }
+}
+
+// State log of generic/solver_main.c:47-66 (write_log): the time, then N
+// doubles per condition (y_0 .. y_{N-1}).  Appends unless append == 0.
+int accelerInt_write_log(const char* path, int append, int NUM, double t, const double* y_host) {
+  const int n = S.n;
+  if (!path || NUM < 0 || (NUM > 0 && !y_host)) {
+    set_err("accelerInt_write_log: bad arguments");
+    return -1;
+  }
+  FILE* fp = fopen(path, append ? "ab" : "wb");
+  if (!fp) {
+    set_err("accelerInt_write_log: cannot open %s", path);
+    return -1;
+  }
+  bool ok = fwrite(&t, sizeof(double), 1, fp) == 1;
+  std::vector<double> rec(n);
+  for (int c = 0; c < NUM && ok; ++c) {
+    for (int i = 0; i < n; ++i) rec[i] = y_host[(size_t)i * NUM + c];
+    ok = fwrite(rec.data(), sizeof(double), n, fp) == (size_t)n;
+  }
+  ok = (fclose(fp) == 0) && ok;
+  if (!ok) set_err("accelerInt_write_log: write to %s failed", path);
+  return ok ? 0 : -1;
+}
+
+// A kinetics network from a binary file: int32 ns, int32 nr, then W[ns],
+// thermo[ns][5], slots int32[nr][6], lnA[nr], beta[nr], EaR[nr] (the arrays of
+// accelerInt_set_kinetics, in that order).
+int accelerInt_load_kinetics(const char* path) {
+  FILE* fp = path ? fopen(path, "rb") : nullptr;
+  if (!fp) {
+    set_err("accelerInt_load_kinetics: cannot open %s", path ? path : "(null)");
+    return -1;
+  }
+  int32_t hdr[2] = {0, 0};
+  bool ok = fread(hdr, sizeof(int32_t), 2, fp) == 2 && hdr[0] > 0 && hdr[0] < 4096 && hdr[1] > 0 &&
+            hdr[1] <= kKinMaxReactions;
+  const int ns = hdr[0], nr = hdr[1];
+  std::vector<double> W, th, lnA, beta, EaR;
+  std::vector<int32_t> sl;
+  if (ok) {
+    W.resize(ns);
+    th.resize((size_t)5 * ns);
+    sl.resize((size_t)6 * nr);
+    lnA.resize(nr);
+    beta.resize(nr);
+    EaR.resize(nr);
+    ok = fread(W.data(), 8, W.size(), fp) == W.size() && fread(th.data(), 8, th.size(), fp) == th.size() &&
+         fread(sl.data(), 4, sl.size(), fp) == sl.size() && fread(lnA.data(), 8, lnA.size(), fp) == lnA.size() &&
+         fread(beta.data(), 8, beta.size(), fp) == beta.size() && fread(EaR.data(), 8, EaR.size(), fp) == EaR.size();
+  }
+  fclose(fp);
+  if (!ok) {
+    set_err("accelerInt_load_kinetics: %s is not a kinetics network file", path);
+    return -1;
+  }
+  return accelerInt_set_kinetics(ns, nr, W.data(), th.data(), sl.data(), lnA.data(), beta.data(), EaR.data());
+}
+
 int accelerInt_set_tolerances(double atol, double rtol) {
   if (!(atol > 0) || !(rtol > 0)) {
     set_err("tolerances must be positive");

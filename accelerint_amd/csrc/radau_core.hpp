@@ -321,8 +321,7 @@ __device__ __forceinline__ double Grp<G>::sum(double v) {
 // branch-free and each pivot-row entry is broadcast just before the column
 // update that consumes it.
 template <int N, int G, int R, class T>
-__device__ __forceinline__ void lu_real(T (&a)[R][N], int (&pos)[R], Perm<N>& perm,
-                                        T (&rd)[R], int& info) {
+__device__ __forceinline__ void lu_init(int (&pos)[R], T (&rd)[R], int& info) {
   const int ln = Grp<G>::lane();
 #pragma unroll
   for (int q = 0; q < R; ++q) {
@@ -330,142 +329,154 @@ __device__ __forceinline__ void lu_real(T (&a)[R][N], int (&pos)[R], Perm<N>& pe
     rd[q] = T(0);
   }
   info = 0;
+}
+
+// Elimination step k of the real LU (k a compile-time constant after unrolling).
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void lu_real_step(const int k, T (&a)[R][N], int (&pos)[R], Perm<N>& perm,
+                                             T (&rd)[R], int& info) {
+  const int ln = Grp<G>::lane();
+  T bv = T(-1);
+  int bkey = 0x7fffffff;
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    T bv = T(-1);
-    int bkey = 0x7fffffff;
+  for (int q = 0; q < R; ++q) {
+    const bool act = pos[q] >= k;
+    const T v = fabs(a[q][k]);
+    const int key = (pos[q] << 8) | Rows<N, G, R>::row(ln, q);
+    const bool better = act && (v > bv || (v == bv && key < bkey));
+    bv = better ? v : bv;
+    bkey = better ? key : bkey;
+  }
+  group_argmax<G>(bv, bkey);
+  const int prow = bkey & 0xff;
+  const int ppos = bkey >> 8;
+  const int plane = prow % G;
+  const int pq = prow / G;
+  perm.set(k, prow);
+  T mine[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const bool act = pos[q] >= k;
-      const T v = fabs(a[q][k]);
-      const int key = (pos[q] << 8) | Rows<N, G, R>::row(ln, q);
-      const bool better = act && (v > bv || (v == bv && key < bkey));
-      bv = better ? v : bv;
-      bkey = better ? key : bkey;
+  for (int q = 0; q < R; ++q) mine[q] = a[q][k];
+  const T piv = Grp<G>::bcast(sel<R>(mine, pq), plane);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
+    pos[q] = is_piv ? k : (pos[q] == k ? ppos : pos[q]);  // LAPACK swaps rows k and ppos
+  }
+  const bool nz = piv != T(0);
+  if (!nz && info == 0) info = k + 1;
+  // LAPACK divides instead of scaling by the reciprocal when |pivot| < sfmin
+  // (FP64 path only; the FP32 factors need no such care)
+  const bool tiny = std::is_same<T, double>::value && fabs(piv) < min_normal<T>();
+  const T rp = recip(piv);
+  T l[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    rd[q] = (Rows<N, G, R>::row(ln, q) == prow) ? (nz ? rp : T(0)) : rd[q];
+    const bool act = nz && pos[q] > k;
+    const T lq = tiny ? a[q][k] / piv : a[q][k] * rp;
+    l[q] = act ? lq : T(0);
+    a[q][k] = act ? lq : a[q][k];
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if (j > k) {
+      T m2[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) m2[q] = a[q][j];
+      const T uj = Grp<G>::bcast(sel<R>(m2, pq), plane);
+#pragma unroll
+      for (int q = 0; q < R; ++q) a[q][j] = fma(-l[q], uj, a[q][j]);
     }
-    group_argmax<G>(bv, bkey);
-    const int prow = bkey & 0xff;
-    const int ppos = bkey >> 8;
-    const int plane = prow % G;
-    const int pq = prow / G;
-    perm.set(k, prow);
-    T mine[R];
+  }
+}
+
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void lu_real(T (&a)[R][N], int (&pos)[R], Perm<N>& perm,
+                                        T (&rd)[R], int& info) {
+  lu_init<N, G, R, T>(pos, rd, info);
 #pragma unroll
-    for (int q = 0; q < R; ++q) mine[q] = a[q][k];
-    const T piv = Grp<G>::bcast(sel<R>(mine, pq), plane);
+  for (int k = 0; k < N; ++k) lu_real_step<N, G, R, T>(k, a, pos, perm, rd, info);
+}
+
+// ---- complex LU (zgetf2 semantics) -------------------------------------------
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void lu_cplx_step(const int k, T (&ar)[R][N], T (&ai)[R][N], int (&pos)[R],
+                                             Perm<N>& perm, T (&rdr)[R], T (&rdi)[R], int& info) {
+  const int ln = Grp<G>::lane();
+  T bv = T(-1);
+  int bkey = 0x7fffffff;
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
-      pos[q] = is_piv ? k : (pos[q] == k ? ppos : pos[q]);  // LAPACK swaps rows k and ppos
-    }
-    const bool nz = piv != T(0);
-    if (!nz && info == 0) info = k + 1;
-    // LAPACK divides instead of scaling by the reciprocal when |pivot| < sfmin
-    // (FP64 path only; the FP32 factors need no such care)
-    const bool tiny = std::is_same<T, double>::value && fabs(piv) < min_normal<T>();
-    const T rp = recip(piv);
-    T l[R];
+  for (int q = 0; q < R; ++q) {
+    const bool act = pos[q] >= k;
+    const T v = fabs(ar[q][k]) + fabs(ai[q][k]);
+    const int key = (pos[q] << 8) | Rows<N, G, R>::row(ln, q);
+    const bool better = act && (v > bv || (v == bv && key < bkey));
+    bv = better ? v : bv;
+    bkey = better ? key : bkey;
+  }
+  group_argmax<G>(bv, bkey);
+  const int prow = bkey & 0xff;
+  const int ppos = bkey >> 8;
+  const int plane = prow % G;
+  const int pq = prow / G;
+  perm.set(k, prow);
+  T mr[R], mi[R];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-      rd[q] = (Rows<N, G, R>::row(ln, q) == prow) ? (nz ? rp : T(0)) : rd[q];
-      const bool act = nz && pos[q] > k;
-      const T lq = tiny ? a[q][k] / piv : a[q][k] * rp;
-      l[q] = act ? lq : T(0);
-      a[q][k] = act ? lq : a[q][k];
-    }
+  for (int q = 0; q < R; ++q) { mr[q] = ar[q][k]; mi[q] = ai[q][k]; }
+  const T pr_ = Grp<G>::bcast(sel<R>(mr, pq), plane);
+  const T pi_ = Grp<G>::bcast(sel<R>(mi, pq), plane);
 #pragma unroll
-    for (int j = 0; j < N; ++j) {
-      if (j > k) {
-        T m2[R];
+  for (int q = 0; q < R; ++q) {
+    const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
+    pos[q] = is_piv ? k : (pos[q] == k ? ppos : pos[q]);
+  }
+  const bool nz = (pr_ != T(0) || pi_ != T(0));
+  if (!nz && info == 0) info = k + 1;
+  T rr, ri;
+  crecip(pr_, pi_, rr, ri);
+  T lr[R], li[R];
 #pragma unroll
-        for (int q = 0; q < R; ++q) m2[q] = a[q][j];
-        const T uj = Grp<G>::bcast(sel<R>(m2, pq), plane);
+  for (int q = 0; q < R; ++q) {
+    const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
+    rdr[q] = is_piv ? (nz ? rr : T(0)) : rdr[q];
+    rdi[q] = is_piv ? (nz ? ri : T(0)) : rdi[q];
+    const bool act = nz && pos[q] > k;
+    const T xr = ar[q][k], xi = ai[q][k];
+    const T vr = xr * rr - xi * ri;
+    const T vi = xr * ri + xi * rr;
+    lr[q] = act ? vr : T(0);
+    li[q] = act ? vi : T(0);
+    ar[q][k] = act ? vr : ar[q][k];
+    ai[q][k] = act ? vi : ai[q][k];
+  }
 #pragma unroll
-        for (int q = 0; q < R; ++q) a[q][j] = fma(-l[q], uj, a[q][j]);
+  for (int j = 0; j < N; ++j) {
+    if (j > k) {
+      T m2r[R], m2i[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) { m2r[q] = ar[q][j]; m2i[q] = ai[q][j]; }
+      const T ur = Grp<G>::bcast(sel<R>(m2r, pq), plane);
+      const T ui = Grp<G>::bcast(sel<R>(m2i, pq), plane);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        ar[q][j] = fma(-lr[q], ur, ar[q][j]);
+        ar[q][j] = fma(li[q], ui, ar[q][j]);
+        ai[q][j] = fma(-lr[q], ui, ai[q][j]);
+        ai[q][j] = fma(-li[q], ur, ai[q][j]);
       }
     }
   }
 }
 
-// ---- complex LU (zgetf2 semantics) -------------------------------------------
 template <int N, int G, int R, class T>
 __device__ __forceinline__ void lu_cplx(T (&ar)[R][N], T (&ai)[R][N], int (&pos)[R],
                                         Perm<N>& perm, T (&rdr)[R], T (&rdi)[R],
                                         int& info) {
-  const int ln = Grp<G>::lane();
+  lu_init<N, G, R, T>(pos, rdr, info);
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    pos[q] = Rows<N, G, R>::valid(ln, q) ? Rows<N, G, R>::row(ln, q) : -1;
-    rdr[q] = T(0);
-    rdi[q] = T(0);
-  }
-  info = 0;
+  for (int q = 0; q < R; ++q) rdi[q] = T(0);
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    T bv = T(-1);
-    int bkey = 0x7fffffff;
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const bool act = pos[q] >= k;
-      const T v = fabs(ar[q][k]) + fabs(ai[q][k]);
-      const int key = (pos[q] << 8) | Rows<N, G, R>::row(ln, q);
-      const bool better = act && (v > bv || (v == bv && key < bkey));
-      bv = better ? v : bv;
-      bkey = better ? key : bkey;
-    }
-    group_argmax<G>(bv, bkey);
-    const int prow = bkey & 0xff;
-    const int ppos = bkey >> 8;
-    const int plane = prow % G;
-    const int pq = prow / G;
-    perm.set(k, prow);
-    T mr[R], mi[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) { mr[q] = ar[q][k]; mi[q] = ai[q][k]; }
-    const T pr_ = Grp<G>::bcast(sel<R>(mr, pq), plane);
-    const T pi_ = Grp<G>::bcast(sel<R>(mi, pq), plane);
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
-      pos[q] = is_piv ? k : (pos[q] == k ? ppos : pos[q]);
-    }
-    const bool nz = (pr_ != T(0) || pi_ != T(0));
-    if (!nz && info == 0) info = k + 1;
-    T rr, ri;
-    crecip(pr_, pi_, rr, ri);
-    T lr[R], li[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const bool is_piv = Rows<N, G, R>::row(ln, q) == prow;
-      rdr[q] = is_piv ? (nz ? rr : T(0)) : rdr[q];
-      rdi[q] = is_piv ? (nz ? ri : T(0)) : rdi[q];
-      const bool act = nz && pos[q] > k;
-      const T xr = ar[q][k], xi = ai[q][k];
-      const T vr = xr * rr - xi * ri;
-      const T vi = xr * ri + xi * rr;
-      lr[q] = act ? vr : T(0);
-      li[q] = act ? vi : T(0);
-      ar[q][k] = act ? vr : ar[q][k];
-      ai[q][k] = act ? vi : ai[q][k];
-    }
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      if (j > k) {
-        T m2r[R], m2i[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) { m2r[q] = ar[q][j]; m2i[q] = ai[q][j]; }
-        const T ur = Grp<G>::bcast(sel<R>(m2r, pq), plane);
-        const T ui = Grp<G>::bcast(sel<R>(m2i, pq), plane);
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-          ar[q][j] = fma(-lr[q], ur, ar[q][j]);
-          ar[q][j] = fma(li[q], ui, ar[q][j]);
-          ai[q][j] = fma(-lr[q], ui, ai[q][j]);
-          ai[q][j] = fma(-li[q], ur, ai[q][j]);
-        }
-      }
-    }
-  }
+  for (int k = 0; k < N; ++k) lu_cplx_step<N, G, R, T>(k, ar, ai, pos, perm, rdr, rdi, info);
 }
 
 // ---- triangular solves (dgetrs / zgetrs, nrhs = 1) ---------------------------
@@ -474,33 +485,84 @@ __device__ __forceinline__ void lu_cplx(T (&ar)[R][N], T (&ai)[R][N], int (&pos)
 // pivot order, as reference BLAS dtrsm does it; rows outside the current
 // step's range take a zero coefficient.
 template <int N, int G, int R, class T>
+__device__ __forceinline__ void fwd_real_step(const int k, const T (&a)[R][N], const int (&pos)[R],
+                                              const Perm<N>& perm, T (&b)[R]) {
+  const int prow = perm.get(k);
+  const T yk = Grp<G>::bcast(sel<R>(b, prow / G), prow % G);
+#pragma unroll
+  for (int q = 0; q < R; ++q) b[q] = fma(-(pos[q] > k ? a[q][k] : T(0)), yk, b[q]);
+}
+
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void bwd_real_step(const int k, const T (&a)[R][N], const int (&pos)[R],
+                                              const Perm<N>& perm, const T (&rd)[R], T (&b)[R], T (&x)[R]) {
+  const int ln = Grp<G>::lane();
+  const int prow = perm.get(k);
+  const int pq = prow / G;
+  const T xk = Grp<G>::bcast(sel<R>(b, pq) * sel<R>(rd, pq), prow % G);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    b[q] = fma(-((pos[q] >= 0 && pos[q] < k) ? a[q][k] : T(0)), xk, b[q]);
+    x[q] = (Rows<N, G, R>::row(ln, q) == k) ? xk : x[q];
+  }
+}
+
+template <int N, int G, int R, class T>
 __device__ __forceinline__ void solve_real(const T (&a)[R][N], const int (&pos)[R],
                                            const Perm<N>& perm, const T (&rd)[R],
                                            T (&b)[R]) {
-  const int ln = Grp<G>::lane();
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const int prow = perm.get(k);
-    const T yk = Grp<G>::bcast(sel<R>(b, prow / G), prow % G);
-#pragma unroll
-    for (int q = 0; q < R; ++q) b[q] = fma(-(pos[q] > k ? a[q][k] : T(0)), yk, b[q]);
-  }
+  for (int k = 0; k < N; ++k) fwd_real_step<N, G, R, T>(k, a, pos, perm, b);
   T x[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) x[q] = T(0);
 #pragma unroll
-  for (int k = N - 1; k >= 0; --k) {
-    const int prow = perm.get(k);
-    const int pq = prow / G;
-    const T xk = Grp<G>::bcast(sel<R>(b, pq) * sel<R>(rd, pq), prow % G);
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      b[q] = fma(-((pos[q] >= 0 && pos[q] < k) ? a[q][k] : T(0)), xk, b[q]);
-      x[q] = (Rows<N, G, R>::row(ln, q) == k) ? xk : x[q];
-    }
-  }
+  for (int k = N - 1; k >= 0; --k) bwd_real_step<N, G, R, T>(k, a, pos, perm, rd, b, x);
 #pragma unroll
   for (int q = 0; q < R; ++q) b[q] = x[q];
+}
+
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void fwd_cplx_step(const int k, const T (&ar)[R][N], const T (&ai)[R][N],
+                                              const int (&pos)[R], const Perm<N>& perm, T (&br)[R], T (&bi)[R]) {
+  const int prow = perm.get(k);
+  const int pl = prow % G, pq = prow / G;
+  const T yr = Grp<G>::bcast(sel<R>(br, pq), pl);
+  const T yi = Grp<G>::bcast(sel<R>(bi, pq), pl);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const bool act = pos[q] > k;
+    const T lr = act ? ar[q][k] : T(0), li = act ? ai[q][k] : T(0);
+    br[q] = fma(-lr, yr, br[q]);
+    br[q] = fma(li, yi, br[q]);
+    bi[q] = fma(-lr, yi, bi[q]);
+    bi[q] = fma(-li, yr, bi[q]);
+  }
+}
+
+template <int N, int G, int R, class T>
+__device__ __forceinline__ void bwd_cplx_step(const int k, const T (&ar)[R][N], const T (&ai)[R][N],
+                                              const int (&pos)[R], const Perm<N>& perm, const T (&rdr)[R],
+                                              const T (&rdi)[R], T (&br)[R], T (&bi)[R], T (&xr)[R], T (&xi)[R]) {
+  const int ln = Grp<G>::lane();
+  const int prow = perm.get(k);
+  const int pl = prow % G, pq = prow / G;
+  const T vr = sel<R>(br, pq), vi = sel<R>(bi, pq);
+  const T dr = sel<R>(rdr, pq), di = sel<R>(rdi, pq);
+  const T kr = Grp<G>::bcast(vr * dr - vi * di, pl);
+  const T ki = Grp<G>::bcast(vr * di + vi * dr, pl);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const bool act = pos[q] >= 0 && pos[q] < k;
+    const T ur = act ? ar[q][k] : T(0), ui = act ? ai[q][k] : T(0);
+    br[q] = fma(-ur, kr, br[q]);
+    br[q] = fma(ui, ki, br[q]);
+    bi[q] = fma(-ur, ki, bi[q]);
+    bi[q] = fma(-ui, kr, bi[q]);
+    const bool mine = Rows<N, G, R>::row(ln, q) == k;
+    xr[q] = mine ? kr : xr[q];
+    xi[q] = mine ? ki : xi[q];
+  }
 }
 
 template <int N, int G, int R, class T>
@@ -508,47 +570,13 @@ __device__ __forceinline__ void solve_cplx(const T (&ar)[R][N], const T (&ai)[R]
                                            const int (&pos)[R], const Perm<N>& perm,
                                            const T (&rdr)[R], const T (&rdi)[R],
                                            T (&br)[R], T (&bi)[R]) {
-  const int ln = Grp<G>::lane();
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const int prow = perm.get(k);
-    const int pl = prow % G, pq = prow / G;
-    const T yr = Grp<G>::bcast(sel<R>(br, pq), pl);
-    const T yi = Grp<G>::bcast(sel<R>(bi, pq), pl);
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const bool act = pos[q] > k;
-      const T lr = act ? ar[q][k] : T(0), li = act ? ai[q][k] : T(0);
-      br[q] = fma(-lr, yr, br[q]);
-      br[q] = fma(li, yi, br[q]);
-      bi[q] = fma(-lr, yi, bi[q]);
-      bi[q] = fma(-li, yr, bi[q]);
-    }
-  }
+  for (int k = 0; k < N; ++k) fwd_cplx_step<N, G, R, T>(k, ar, ai, pos, perm, br, bi);
   T xr[R], xi[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) { xr[q] = T(0); xi[q] = T(0); }
 #pragma unroll
-  for (int k = N - 1; k >= 0; --k) {
-    const int prow = perm.get(k);
-    const int pl = prow % G, pq = prow / G;
-    const T vr = sel<R>(br, pq), vi = sel<R>(bi, pq);
-    const T dr = sel<R>(rdr, pq), di = sel<R>(rdi, pq);
-    const T kr = Grp<G>::bcast(vr * dr - vi * di, pl);
-    const T ki = Grp<G>::bcast(vr * di + vi * dr, pl);
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const bool act = pos[q] >= 0 && pos[q] < k;
-      const T ur = act ? ar[q][k] : T(0), ui = act ? ai[q][k] : T(0);
-      br[q] = fma(-ur, kr, br[q]);
-      br[q] = fma(ui, ki, br[q]);
-      bi[q] = fma(-ur, ki, bi[q]);
-      bi[q] = fma(-ui, kr, bi[q]);
-      const bool mine = Rows<N, G, R>::row(ln, q) == k;
-      xr[q] = mine ? kr : xr[q];
-      xi[q] = mine ? ki : xi[q];
-    }
-  }
+  for (int k = N - 1; k >= 0; --k) bwd_cplx_step<N, G, R, T>(k, ar, ai, pos, perm, rdr, rdi, br, bi, xr, xi);
 #pragma unroll
   for (int q = 0; q < R; ++q) { br[q] = xr[q]; bi[q] = xi[q]; }
 }
@@ -1042,6 +1070,8 @@ struct Radau {
           E2i[q][j] = (MT)((d ? g2i : 0.0) * s2);
         }
       }
+      // (interleaving the two factorizations' steps was measured slower at
+      // N = 54: the larger live set spills)
       lu_real<N, G, R, MT>(E1, pos1, perm1, rd1, info);
       lu_cplx<N, G, R, MT>(E2r, E2i, pos2, perm2, rd2r, rd2i, info2);
     }
@@ -1203,7 +1233,9 @@ struct Radau {
           DZ2[q] = rkTiAi10 * x1 + rkTiAi11 * x2 + rkTiAi12 * x3;
           DZ3[q] = rkTiAi20 * x1 + rkTiAi21 * x2 + rkTiAi22 * x3;
         }
-          solve1(DZ1);
+        // (interleaving the real and complex substitutions was measured
+        // slower in this kernel: the larger live set spills)
+        solve1(DZ1);
         solve2(DZ2, DZ3);
 #pragma unroll
         for (int q = 0; q < R; ++q) {

@@ -57,6 +57,16 @@ class KineticsMechanism:
     def save(self, path: str) -> None:
         np.savez(path, **{f.name: getattr(self, f.name) for f in dataclasses.fields(self)})
 
+    def write_binary(self, path: str) -> None:
+        """The network file ``accelerInt_load_kinetics`` reads (and the command-line
+        driver's ``--mech``): int32 ns, int32 nr, W, thermo, slots (int32), lnA,
+        beta, EaR."""
+        with open(path, "wb") as f:
+            np.array([self.ns, self.nr], dtype=np.int32).tofile(f)
+            for a, t in ((self.W, np.float64), (self.thermo, np.float64), (self.slots, np.int32),
+                         (self.lnA, np.float64), (self.beta, np.float64), (self.EaR, np.float64)):
+                np.ascontiguousarray(a, dtype=t).tofile(f)
+
     @classmethod
     def load(cls, path: str) -> "KineticsMechanism":
         with np.load(path, allow_pickle=False) as z:

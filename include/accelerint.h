@@ -84,6 +84,23 @@ int accelerInt_set_kinetics(int ns, int nr, const double *W, const double *therm
                             const int *slots, const double *lnA, const double *beta,
                             const double *EaR);
 
+/* ---- files of the reference's command-line driver -------------------------
+ * Initial conditions (generic/read_initial_conditions.c:30-78, CONP): NUM
+ * records of N + 2 doubles [t, y_0, param, y_1 .. y_{N-1}], N =
+ * accelerInt_num_states() of the selected mechanism; y_host column-major
+ * (y_host[i*NUM + c]), var_host[c] = param.  The reference's signature
+ * allocates and exits on error; the accelerInt_ variant fills caller arrays and
+ * returns 0 or -1 (message: accelerInt_last_error). */
+void read_initial_conditions(const char *filename, int NUM, double **y_host, double **variable_host);
+int accelerInt_read_initial_conditions(const char *filename, int NUM, double *y_host, double *var_host);
+/* State log (generic/solver_main.c:47-66): t, then N doubles per condition;
+ * appends unless append == 0.  Returns 0 or -1. */
+int accelerInt_write_log(const char *path, int append, int NUM, double t, const double *y_host);
+/* accelerInt_set_kinetics from a binary network file: int32 ns, int32 nr,
+ * W[ns], thermo[ns][5], slots int32[nr][6], lnA[nr], beta[nr], EaR[nr].
+ * Returns 0 or -1. */
+int accelerInt_load_kinetics(const char *path);
+
 /* Runtime tolerances (the reference's ATOL/RTOL macros; defaults 1e-10, 1e-6)
  * and the CONST_TIME_STEP switch. */
 int accelerInt_set_tolerances(double atol, double rtol);
