@@ -50,6 +50,7 @@ EXPORTS = (
     "accelerInt_read_log", "accelerInt_graph_capture", "accelerInt_graph_launch", "accelerInt_graph_release",
     "accelerInt_set_schedule", "accelerInt_get_schedule", "read_initial_conditions",
     "accelerInt_read_initial_conditions", "accelerInt_write_log", "accelerInt_load_kinetics",
+    "accelerInt_set_launch_shape", "accelerInt_get_launch_shape", "accelerInt_autotune", "accelerInt_load_tuning",
 )
 
 
@@ -99,6 +100,10 @@ def lib():
         "accelerInt_read_initial_conditions": ([ctypes.c_char_p, i, v, v], i),
         "accelerInt_write_log": ([ctypes.c_char_p, i, i, d, v], i),
         "accelerInt_load_kinetics": ([ctypes.c_char_p], i),
+        "accelerInt_set_launch_shape": ([i, i, i], i),
+        "accelerInt_get_launch_shape": ([i, i], i),
+        "accelerInt_autotune": ([i, d, d, d, v, v, ctypes.c_char_p, v], i),
+        "accelerInt_load_tuning": ([ctypes.c_char_p], i),
         "accelerInt_get_schedule": ([v, v, i], i),
         "accelerInt_mlp_init": ([i, ctypes.c_uint], i),
         "accelerInt_mlp_param_count": ([], i),
@@ -196,6 +201,31 @@ def set_precision(mode: int):
     """PRECISION_FP64 (default) or PRECISION_MIXED (FP32 iteration-matrix LU and
     Newton solves; FP64 residuals, error estimate, step control and state)."""
     _check(lib().accelerInt_set_precision(int(mode)), "accelerInt_set_precision")
+
+
+def set_launch_shape(n_states: int, precision: int, waves_per_simd: int):
+    _check(lib().accelerInt_set_launch_shape(int(n_states), int(precision), int(waves_per_simd)),
+           "accelerInt_set_launch_shape")
+
+
+def get_launch_shape(n_states: int, precision: int) -> int:
+    return _check(lib().accelerInt_get_launch_shape(int(n_states), int(precision)), "accelerInt_get_launch_shape")
+
+
+def autotune(num: int, t_start: float, t_end: float, stepsize: float, y_dev_ptr: int, var_dev_ptr: int,
+             cache_path=None):
+    """Times the launch shapes of the selected system and precision on num
+    device-resident conditions and keeps the fastest: (waves_per_simd, ms per
+    shape)."""
+    ms = np.zeros(3, dtype=np.float64)
+    w = _check(lib().accelerInt_autotune(int(num), float(t_start), float(t_end), float(stepsize), y_dev_ptr,
+                                         var_dev_ptr, os.fsencode(cache_path) if cache_path else None,
+                                         ms.ctypes.data), "accelerInt_autotune")
+    return w, ms
+
+
+def load_tuning(cache_path: str) -> int:
+    return _check(lib().accelerInt_load_tuning(os.fsencode(cache_path)), "accelerInt_load_tuning")
 
 
 def set_schedule(mode: int):

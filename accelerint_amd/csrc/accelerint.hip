@@ -26,7 +26,8 @@
 #include "mech_vdp.hpp"
 #include "radau_core.hpp"
 
-// Waves per SIMD the register allocator targets for each instantiation.
+// Waves per SIMD the register allocator targets (defaults; van der Pol and
+// the 14-state network also have the autotuner's alternatives, kShapes*).
 #ifndef ACC_VDP_WPE
 #define ACC_VDP_WPE 2
 #endif
@@ -437,7 +438,74 @@ struct State {
   MechKinetics14::Params kp14{};
   MechKinetics54::Params kp54{};
   std::string err;
+  // launch shapes (waves per SIMD) by [system: 0 vdP, 1 14-state][precision]
+  int wpe[2][2] = {{ACC_VDP_WPE, ACC_VDP_WPE}, {ACC_K14_WPE, ACC_K14_WPE}};
 } S;
+
+// Launch shapes the autotuner chooses among: waves per SIMD the register
+// allocator targets (512 / WPE registers per lane; fewer waves, fewer spills).
+// The 54-state kernel has one shape: four 54-state groups fill the LDS of a
+// CU and need all 512 registers of a lane, so one wave per SIMD.
+constexpr int kShapesVdp[] = {1, 2, 4};
+constexpr int kShapesK14[] = {1, 2, 3};
+
+static_assert(kShapesVdp[1] == ACC_VDP_WPE && kShapesK14[1] == ACC_K14_WPE, "defaults are the middle shapes");
+
+int shape_system() { return S.mech == ACCELERINT_MECH_VAN_DER_POL ? 0 : (S.n == 14 ? 1 : -1); }
+
+std::string device_key() {
+  hipDeviceProp_t prop;
+  if (S.device < 0 || hipGetDeviceProperties(&prop, S.device) != hipSuccess) return "";
+  char buf[300];
+  snprintf(buf, sizeof buf, "%s/%dCU", prop.gcnArchName, prop.multiProcessorCount);
+  for (char* c = buf; *c; ++c)
+    if (*c == '"' || *c == '\\') *c = '_';
+  return buf;
+}
+
+// Tuning cache: a JSON array with one object per line,
+//   {"device": "...", "n_states": N, "precision": P, "waves_per_simd": W, "ms": [...]}
+// Lines of other devices, sizes or precisions are kept as they are.
+struct TuneEntry {
+  std::string device;
+  int n = 0, precision = 0, wpe = 0;
+  std::string line;
+};
+
+std::vector<TuneEntry> read_tuning(const char* path) {
+  std::vector<TuneEntry> out;
+  FILE* fp = fopen(path, "r");
+  if (!fp) return out;
+  char buf[2048];
+  while (fgets(buf, sizeof buf, fp)) {
+    TuneEntry e;
+    char dev[512];
+    const char* p = strstr(buf, "{\"device\": \"");
+    if (!p || sscanf(p, "{\"device\": \"%511[^\"]\", \"n_states\": %d, \"precision\": %d, \"waves_per_simd\": %d",
+                     dev, &e.n, &e.precision, &e.wpe) != 4)
+      continue;
+    e.device = dev;
+    e.line = p;
+    while (!e.line.empty() && (e.line.back() == '\n' || e.line.back() == ',' || e.line.back() == ' ' ||
+                               e.line.back() == ']'))
+      e.line.pop_back();
+    out.push_back(e);
+  }
+  fclose(fp);
+  return out;
+}
+
+bool write_tuning(const char* path, const std::vector<TuneEntry>& entries) {
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* fp = fopen(tmp.c_str(), "w");
+  if (!fp) return false;
+  fprintf(fp, "[\n");
+  for (size_t i = 0; i < entries.size(); ++i)
+    fprintf(fp, "%s%s\n", entries[i].line.c_str(), i + 1 < entries.size() ? "," : "");
+  fprintf(fp, "]\n");
+  if (fclose(fp) != 0) return false;
+  return rename(tmp.c_str(), path) == 0;
+}
 
 void set_err(const char* fmt, ...) {
   char buf[512];
@@ -533,15 +601,29 @@ template <class MT>
 hipError_t launch_mt(int num, double t_start, double t_end, double step, const double* pr, double* y,
                      int* codes, uint32_t* cnt, const int* list, const int* num_dev, int* esc,
                      hipStream_t s, float* cost = nullptr) {
+  constexpr int prec = std::is_same<MT, float>::value ? 1 : 0;
   switch (S.mech) {
-    case ACCELERINT_MECH_VAN_DER_POL:
-      return launch_radau<MechVdP, 1, 2, 256, ACC_VDP_WPE, MT>(num, t_start, t_end, step, pr, y, codes, cnt,
-                                                               S.d_work, list, num_dev, esc, S.P,
-                                                               MechVdP::Params{}, s, cost);
+    case ACCELERINT_MECH_VAN_DER_POL: {
+#define ACC_VDP_LAUNCH(W)                                                                                     \
+  launch_radau<MechVdP, 1, 2, 256, W, MT>(num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, \
+                                          esc, S.P, MechVdP::Params{}, s, cost)
+      const int w = S.wpe[0][prec];
+      if (w == kShapesVdp[0]) return ACC_VDP_LAUNCH(kShapesVdp[0]);
+      if (w == kShapesVdp[2]) return ACC_VDP_LAUNCH(kShapesVdp[2]);
+      return ACC_VDP_LAUNCH(kShapesVdp[1]);
+#undef ACC_VDP_LAUNCH
+    }
     case ACCELERINT_MECH_KINETICS:
-      if (S.n == 14)
-        return launch_radau<MechKinetics14, 16, 1, 128, ACC_K14_WPE, MT>(
-            num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp14, s, cost);
+      if (S.n == 14) {
+#define ACC_K14_LAUNCH(W)                                                                                  \
+  launch_radau<MechKinetics14, 16, 1, 128, W, MT>(num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, \
+                                                  num_dev, esc, S.P, S.kp14, s, cost)
+        const int w = S.wpe[1][prec];
+        if (w == kShapesK14[0]) return ACC_K14_LAUNCH(kShapesK14[0]);
+        if (w == kShapesK14[2]) return ACC_K14_LAUNCH(kShapesK14[2]);
+        return ACC_K14_LAUNCH(kShapesK14[1]);
+#undef ACC_K14_LAUNCH
+      }
       if (S.n == 54 && S.learned)
         return launch_radau<MechKinetics54, 64, 1, 256, ACC_K54_WPE, MT, true>(
             num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P, S.kp54, s, cost);
@@ -926,6 +1008,130 @@ int accelerInt_get_schedule(float* work, int* order, int num) {
   return ready ? num : 0;
 }
 
+int accelerInt_set_launch_shape(int n_states, int precision, int waves_per_simd) {
+  const int sys = n_states == 2 ? 0 : (n_states == 14 ? 1 : -1);
+  const int* shapes = sys == 0 ? kShapesVdp : kShapesK14;
+  if (sys < 0 || (precision != 0 && precision != 1) ||
+      (waves_per_simd != shapes[0] && waves_per_simd != shapes[1] && waves_per_simd != shapes[2])) {
+    set_err("accelerInt_set_launch_shape: no shape with %d waves per SIMD for %d states", waves_per_simd, n_states);
+    return -1;
+  }
+  S.wpe[sys][precision] = waves_per_simd;
+  return 0;
+}
+
+int accelerInt_get_launch_shape(int n_states, int precision) {
+  if (n_states == 54) return ACC_K54_WPE;
+  const int sys = n_states == 2 ? 0 : (n_states == 14 ? 1 : -1);
+  if (sys < 0 || (precision != 0 && precision != 1)) {
+    set_err("accelerInt_get_launch_shape: unknown system size or precision");
+    return -1;
+  }
+  return S.wpe[sys][precision];
+}
+
+// Times every launch shape of the selected system and precision on the given
+// device-resident conditions (copied into the library's own state buffer, so
+// y_dev is left untouched), keeps the fastest, and records it in cache_path
+// (if given).  One untimed run per shape first, so module loading is not timed.
+int accelerInt_autotune(int NUM, double t_start, double t_end, double stepsize, const double* y_dev,
+                        const double* var_dev, const char* cache_path, double* ms_out) {
+  if (!ensure_init()) return -1;
+  if (NUM <= 0 || NUM > S.padded || !y_dev || !var_dev) {
+    set_err("accelerInt_autotune: bad arguments (NUM must be 1..%d)", S.padded);
+    return -1;
+  }
+  const int sys = shape_system();
+  const int prec = S.precision == ACCELERINT_PRECISION_MIXED ? 1 : 0;
+  if (sys < 0) return ACC_K54_WPE;  // a single shape
+  const int* shapes = sys == 0 ? kShapesVdp : kShapesK14;
+  const int keep = S.wpe[sys][prec];
+  const size_t bytes = (size_t)NUM * S.n * sizeof(double);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!HIPOK(hipEventCreate(&e0)) || !HIPOK(hipEventCreate(&e1))) return -1;
+  double ms[3] = {0, 0, 0};
+  int rc = 0;
+  for (int k = 0; k < 3 && rc == 0; ++k) {
+    S.wpe[sys][prec] = shapes[k];
+    for (int pass = 0; pass < 2 && rc == 0; ++pass) {
+      const int m = pass == 0 ? (NUM < 64 ? NUM : 64) : NUM;
+      if (!HIPOK(hipMemcpyAsync(S.d_y, y_dev, bytes, hipMemcpyDeviceToDevice, S.stream)) ||
+          !HIPOK(hipEventRecord(e0, S.stream))) {
+        rc = -1;
+        break;
+      }
+      rc = run_steps(m, t_start, t_end, stepsize, S.d_y, var_dev, S.d_codes, nullptr, S.stream);
+      float t = 0.0f;
+      if (rc == 0 && (!HIPOK(hipEventRecord(e1, S.stream)) || !HIPOK(hipEventSynchronize(e1)) ||
+                      !HIPOK(hipEventElapsedTime(&t, e0, e1))))
+        rc = -1;
+      ms[k] = t;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  S.cost_num = S.order_num = 0;  // the timing runs' work does not describe the caller's next batch
+  if (rc != 0) {
+    S.wpe[sys][prec] = keep;
+    return -1;
+  }
+  int best = 0;
+  for (int k = 1; k < 3; ++k)
+    if (ms[k] < ms[best]) best = k;
+  S.wpe[sys][prec] = shapes[best];
+  if (ms_out)
+    for (int k = 0; k < 3; ++k) ms_out[k] = ms[k];
+  if (cache_path) {
+    std::vector<TuneEntry> entries = read_tuning(cache_path);
+    const std::string dev = device_key();
+    TuneEntry e;
+    e.device = dev;
+    e.n = S.n;
+    e.precision = prec;
+    e.wpe = shapes[best];
+    char buf[1024];
+    snprintf(buf, sizeof buf,
+             "{\"device\": \"%s\", \"n_states\": %d, \"precision\": %d, \"waves_per_simd\": %d, "
+             "\"conditions\": %d, \"shapes\": [%d, %d, %d], \"ms\": [%.3f, %.3f, %.3f]}",
+             dev.c_str(), S.n, prec, shapes[best], NUM, shapes[0], shapes[1], shapes[2], ms[0], ms[1], ms[2]);
+    e.line = buf;
+    bool replaced = false;
+    for (auto& x : entries)
+      if (x.device == dev && x.n == e.n && x.precision == prec) {
+        x = e;
+        replaced = true;
+      }
+    if (!replaced) entries.push_back(e);
+    if (!write_tuning(cache_path, entries)) {
+      set_err("accelerInt_autotune: cannot write %s", cache_path);
+      return -1;
+    }
+  }
+  return shapes[best];
+}
+
+// Applies the cache entries of this device (any system size and precision).
+// Returns the number of entries applied, or -1.
+int accelerInt_load_tuning(const char* cache_path) {
+  if (!ensure_init()) return -1;
+  if (!cache_path) {
+    set_err("accelerInt_load_tuning: no path");
+    return -1;
+  }
+  const std::string dev = device_key();
+  int applied = 0;
+  for (const TuneEntry& e : read_tuning(cache_path)) {
+    if (e.device != dev) continue;
+    const int sys = e.n == 2 ? 0 : (e.n == 14 ? 1 : -1);
+    if (sys < 0 || (e.precision != 0 && e.precision != 1)) continue;
+    const int* shapes = sys == 0 ? kShapesVdp : kShapesK14;
+    if (e.wpe != shapes[0] && e.wpe != shapes[1] && e.wpe != shapes[2]) continue;
+    S.wpe[sys][e.precision] = e.wpe;
+    ++applied;
+  }
+  return applied;
+}
+
 int accelerInt_set_learned_start(int on) {
   if (on) {
     const MlpView v = mlp_view();
@@ -1053,6 +1259,8 @@ void accelerInt_initialize(int NUM, int device) {
     return;
   }
   S.have_status = 0;
+  // launch shapes recorded by accelerInt_autotune for this device
+  if (const char* cache = getenv("ACCELERINT_TUNE_CACHE")) accelerInt_load_tuning(cache);
 }
 
 // solver_interface.cu:225-275, with the batches of `padded` conditions kept and

@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain-FP64 comparison leg of a mixed-precision run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--schedule", choices=["longest-first", "index"], default="longest-first",
+                    help="queue order of the persistent kernel (results do not depend on it)")
     ap.add_argument("--train-samples", type=int, default=2000000,
                     help="learned-start configs: logged samples to train on (first accepted step of each "
                          "condition and global step, from an untimed run)")
@@ -183,6 +185,7 @@ def main():
     acc.set_tolerances(1e-10, 1e-6)
     acc.set_abort_on_error(False)
     acc.initialize(num, 0)
+    acc.set_schedule(acc.SCHEDULE_INDEX if args.schedule == "index" else acc.SCHEDULE_LONGEST_FIRST)
 
     dev = torch.device("cuda:0")
     # one explicit (default-priority) stream for the library's launches and the
@@ -242,6 +245,16 @@ def main():
     precision = args.precision or DEFAULT_PRECISION[args.config]
     learned_on = args.config in LEARNED
     log("config %s: %s" % (args.config, desc))
+    # untimed: the autotuner picks the launch shape (waves per SIMD) of this
+    # system and precision on the workload's own conditions (2 global steps)
+    acc.set_precision(acc.PRECISION_MIXED if precision == "mixed" else acc.PRECISION_FP64)
+    y_tune = torch.from_numpy(np.ascontiguousarray(y0.T)).to(dev)
+    torch.cuda.synchronize()
+    w_best, ms_shapes = acc.autotune(num, 0.0, 2 * dt, dt, y_tune.data_ptr(), pr_dev.data_ptr())
+    del y_tune
+    shape = {"waves_per_simd": w_best,
+             "tuned_ms": [round(float(x), 3) for x in ms_shapes] if n != 54 else None}
+    log("launch shape: %s" % json.dumps(shape))
     training = None
     if learned_on:
         # untimed: log the first accepted step of every condition and global
@@ -324,6 +337,8 @@ def main():
         method += " with the learned Newton start of each global step's first step (zero-start fallback)"
     config = {"workload": "%s: %s" % (args.config, desc), "method": method,
               "rtol": 1e-6, "atol": 1e-10, "n_states": n, "conditions": num, "dt": dt}
+    config["launch_shape"] = shape
+    config["schedule"] = args.schedule
     if training:
         config["training"] = training
     if compare:

@@ -115,6 +115,24 @@ int accelerInt_set_const_time_step(int on);
 #define ACCELERINT_PRECISION_MIXED 1
 int accelerInt_set_precision(int mode);
 
+/* Launch shapes (no reference counterpart): the waves per SIMD the integration
+ * kernel's register allocation targets.  van der Pol: 1, 2 (default) or 4; the
+ * 14-state network: 1, 2 (default) or 3; the 54-state network: 1 only.
+ * accelerInt_autotune times each shape of the selected system and precision on
+ * NUM device-resident conditions (y_dev condition-major, left unchanged),
+ * keeps the fastest and, if cache_path is given, records it there (a JSON array,
+ * one entry per device, system size and precision); ms_out (may be NULL)
+ * receives the three timings.  Returns the chosen waves per SIMD, or -1.
+ * accelerInt_load_tuning applies a cache's entries for this device (returns
+ * how many); accelerInt_initialize does so itself when the environment
+ * variable ACCELERINT_TUNE_CACHE names a cache.  Results do not depend on the
+ * shape. */
+int accelerInt_set_launch_shape(int n_states, int precision, int waves_per_simd);
+int accelerInt_get_launch_shape(int n_states, int precision);
+int accelerInt_autotune(int NUM, double t_start, double t_end, double stepsize, const double *y_dev,
+                        const double *var_dev, const char *cache_path, double *ms_out);
+int accelerInt_load_tuning(const char *cache_path);
+
 /* Order in which the persistent kernel takes conditions (no reference
  * counterpart; results do not depend on it).  LONGEST_FIRST (default): when
  * the previous integration covered the same number of conditions in one

@@ -357,3 +357,47 @@ def test_longest_first_schedule(session, net14, net54, n, precision):
     assert np.array_equal(a[0], b[0])
     assert np.array_equal(a[1], b[1])
     assert np.array_equal(a[2], b[2])
+
+
+@pytest.mark.parametrize("system,precision", [("vdp", 0), ("k14", 0), ("k14", 1)])
+def test_launch_shapes_agree_and_autotune(session, net14, tmp_path, system, precision):
+    """Every launch shape gives bit-identical results; the autotuner picks one of
+    them, records it in its JSON cache, and the cache restores it."""
+    import json
+    from hipmem import DeviceArray
+    if system == "vdp":
+        num, steps, dt, n = 2048, 20, 1e-3, 2
+        y0, pr = M.van_der_pol_conditions(num, seed=61)
+        _vdp_setup(num)
+        shapes = (1, 2, 4)
+    else:
+        num, steps, dt, n = 1024, 5, 1e-6, 14
+        y0, pr = M.initial_conditions(net14, num, seed=62)
+        _kin_setup(net14, num)
+        shapes = (1, 2, 3)
+    acc.set_precision(precision)
+    try:
+        res = []
+        for w in shapes:
+            acc.set_launch_shape(n, precision, w)
+            assert acc.get_launch_shape(n, precision) == w
+            y = y0.copy()
+            acc.integrate(num, 0.0, steps * dt, dt, y, pr)
+            res.append((y, acc.get_status(num)))
+        for y, codes in res[1:]:
+            assert np.array_equal(y, res[0][0]) and np.array_equal(codes, res[0][1])
+        yd, pd = DeviceArray(y0.T), DeviceArray(pr)
+        cache = str(tmp_path / "tune.json")
+        w, ms = acc.autotune(num, 0.0, steps * dt, dt, yd.addr, pd.addr, cache)
+        print("%s precision %d: shapes %s ms %s -> %d waves per SIMD" % (system, precision, shapes, ms, w))
+        assert w in shapes and ms.min() > 0 and w == shapes[int(np.argmin(ms))]
+        assert np.array_equal(yd.get(), np.ascontiguousarray(y0.T))  # the caller's states are untouched
+        entries = json.load(open(cache))
+        assert [(e["n_states"], e["precision"], e["waves_per_simd"]) for e in entries] == [(n, precision, w)]
+        other = [s for s in shapes if s != w][0]
+        acc.set_launch_shape(n, precision, other)
+        assert acc.load_tuning(cache) == 1
+        assert acc.get_launch_shape(n, precision) == w
+    finally:
+        acc.set_launch_shape(n, precision, shapes[1])
+        acc.set_precision(acc.PRECISION_FP64)
