@@ -27,6 +27,8 @@ MECH_VAN_DER_POL = 0
 MECH_KINETICS = 2
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
+JACOBIAN_ANALYTIC = 0
+JACOBIAN_FINITE_DIFFERENCE = 1
 SCHEDULE_INDEX = 0
 SCHEDULE_LONGEST_FIRST = 1
 NCNT = 10
@@ -51,6 +53,7 @@ EXPORTS = (
     "accelerInt_set_schedule", "accelerInt_get_schedule", "read_initial_conditions",
     "accelerInt_read_initial_conditions", "accelerInt_write_log", "accelerInt_load_kinetics",
     "accelerInt_set_launch_shape", "accelerInt_get_launch_shape", "accelerInt_autotune", "accelerInt_load_tuning",
+    "accelerInt_set_jacobian",
 )
 
 
@@ -101,6 +104,7 @@ def lib():
         "accelerInt_write_log": ([ctypes.c_char_p, i, i, d, v], i),
         "accelerInt_load_kinetics": ([ctypes.c_char_p], i),
         "accelerInt_set_launch_shape": ([i, i, i], i),
+        "accelerInt_set_jacobian": ([i], i),
         "accelerInt_get_launch_shape": ([i, i], i),
         "accelerInt_autotune": ([i, d, d, d, v, v, ctypes.c_char_p, v], i),
         "accelerInt_load_tuning": ([ctypes.c_char_p], i),
@@ -201,6 +205,12 @@ def set_precision(mode: int):
     """PRECISION_FP64 (default) or PRECISION_MIXED (FP32 iteration-matrix LU and
     Newton solves; FP64 residuals, error estimate, step control and state)."""
     _check(lib().accelerInt_set_precision(int(mode)), "accelerInt_set_precision")
+
+
+def set_jacobian(mode: int):
+    """JACOBIAN_ANALYTIC (default) or JACOBIAN_FINITE_DIFFERENCE (generic/fd_jacob.c;
+    van der Pol and the 14-state network)."""
+    _check(lib().accelerInt_set_jacobian(int(mode)), "accelerInt_set_jacobian")
 
 
 def set_launch_shape(n_states: int, precision: int, waves_per_simd: int):

@@ -5,7 +5,8 @@
 // lines ("Time:", "Time per step:", "TFinal:").
 //
 //   accelerint_main [num_IVPs] [device] [--end T] [--step dt] [--mech network.bin]
-//                   [--ics file] [--log] [--log-end-only] [--mixed] [--atol a] [--rtol r]
+//                   [--ics file] [--log] [--log-end-only] [--mixed] [--fd-jacobian]
+//                   [--atol a] [--rtol r]
 //
 // Without --mech the van der Pol example is integrated.  The end time and
 // step (the reference's compile-time end_time / t_step) default to 1e-3 and
@@ -32,7 +33,7 @@ static void usage_exit(const char* msg) {
 int main(int argc, char** argv) {
   int NUM = 1, device = 0, positional = 0;
   double end_time = 1e-3, t_step = 1e-6, atol = 1e-10, rtol = 1e-6;
-  bool log_steps = false, log_end = false, mixed = false;
+  bool log_steps = false, log_end = false, mixed = false, fd = false;
   std::string mech, ics = "ign_data.bin";
   for (int a = 1; a < argc; ++a) {
     const std::string s = argv[a];
@@ -47,6 +48,7 @@ int main(int argc, char** argv) {
     else if (s == "--log") log_steps = true;
     else if (s == "--log-end-only") log_end = true;
     else if (s == "--mixed") mixed = true;
+    else if (s == "--fd-jacobian") fd = true;
     else if (s == "--atol") atol = atof(next());
     else if (s == "--rtol") rtol = atof(next());
     else if (positional == 0) {
@@ -60,7 +62,7 @@ int main(int argc, char** argv) {
       ++positional;
     } else {
       usage_exit("usage: accelerint_main [num_IVPs] [device] [--end T] [--step dt] [--mech file] [--ics file] "
-                 "[--log] [--log-end-only] [--mixed] [--atol a] [--rtol r]");
+                 "[--log] [--log-end-only] [--mixed] [--fd-jacobian] [--atol a] [--rtol r]");
     }
   }
   if (!(end_time > 0) || !(t_step > 0)) usage_exit("end time and step must be positive");
@@ -72,6 +74,7 @@ int main(int argc, char** argv) {
   }
   accelerInt_set_tolerances(atol, rtol);
   if (mixed) accelerInt_set_precision(ACCELERINT_PRECISION_MIXED);
+  if (fd) accelerInt_set_jacobian(ACCELERINT_JACOBIAN_FINITE_DIFFERENCE);
 
   double* y_host = nullptr;
   double* var_host = nullptr;

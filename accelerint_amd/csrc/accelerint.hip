@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "accelerint.h"
+#include "mech_fd.hpp"
 #include "mech_kinetics.hpp"
 #include "mech_vdp.hpp"
 #include "radau_core.hpp"
@@ -395,6 +396,7 @@ struct State {
   int mech = ACCELERINT_MECH_VAN_DER_POL;
   int n = 2;
   int precision = ACCELERINT_PRECISION_FP64;
+  int jacobian = ACCELERINT_JACOBIAN_ANALYTIC;
   int learned = 0;          // learned Newton start on (the network is looked up per launch)
   float* own_log_x = nullptr;  // library-owned training-sample log (accelerInt_start_log)
   float* own_log_z = nullptr;
@@ -602,6 +604,19 @@ hipError_t launch_mt(int num, double t_start, double t_end, double step, const d
                      int* codes, uint32_t* cnt, const int* list, const int* num_dev, int* esc,
                      hipStream_t s, float* cost = nullptr) {
   constexpr int prec = std::is_same<MT, float>::value ? 1 : 0;
+  if (S.jacobian == ACCELERINT_JACOBIAN_FINITE_DIFFERENCE) {
+    // finite-difference Jacobian: van der Pol and the 14-state network
+    if (S.mech == ACCELERINT_MECH_VAN_DER_POL)
+      return launch_radau<MechFD<MechVdP>, 1, 2, 256, ACC_VDP_WPE, MT>(
+          num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P,
+          MechFD<MechVdP>::Params{MechVdP::Params{}, S.P.atol, S.P.rtol}, s, cost);
+    if (S.mech == ACCELERINT_MECH_KINETICS && S.n == 14)
+      return launch_radau<MechFD<MechKinetics14>, 16, 1, 128, ACC_K14_WPE, MT>(
+          num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P,
+          MechFD<MechKinetics14>::Params{S.kp14, S.P.atol, S.P.rtol}, s, cost);
+    g_launch_msg = "the finite-difference Jacobian is compiled for van der Pol and the 14-state network only";
+    return hipErrorInvalidValue;
+  }
   switch (S.mech) {
     case ACCELERINT_MECH_VAN_DER_POL: {
 #define ACC_VDP_LAUNCH(W)                                                                                     \
@@ -973,6 +988,15 @@ int accelerInt_set_precision(int mode) {
     return -1;
   }
   S.precision = mode;
+  return 0;
+}
+
+int accelerInt_set_jacobian(int mode) {
+  if (mode != ACCELERINT_JACOBIAN_ANALYTIC && mode != ACCELERINT_JACOBIAN_FINITE_DIFFERENCE) {
+    set_err("unknown Jacobian mode %d", mode);
+    return -1;
+  }
+  S.jacobian = mode;
   return 0;
 }
 

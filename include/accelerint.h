@@ -115,6 +115,17 @@ int accelerInt_set_const_time_step(int on);
 #define ACCELERINT_PRECISION_MIXED 1
 int accelerInt_set_precision(int mode);
 
+/* Jacobian of the iteration matrices: the mechanism's analytic eval_jacob
+ * (default) or the reference's finite-difference one, generic/fd_jacob.c:24-127
+ * (FD_ORD 1: one forward difference per column, increment
+ * max(sqrt(eps)|y_j|, r0 / (atol + rtol|y_j|)) with r0 = 1000 rtol eps N
+ * sqrt(sum((atol + rtol|y_i|) f_i)^2 / N)), for callbacks without an analytic
+ * Jacobian.  Compiled for van der Pol and the 14-state network (single-network
+ * launches).  May be changed between integrate calls; returns 0 or -1. */
+#define ACCELERINT_JACOBIAN_ANALYTIC 0
+#define ACCELERINT_JACOBIAN_FINITE_DIFFERENCE 1
+int accelerInt_set_jacobian(int mode);
+
 /* Launch shapes (no reference counterpart): the waves per SIMD the integration
  * kernel's register allocation targets.  van der Pol: 1, 2 (default) or 4; the
  * 14-state network: 1, 2 (default) or 3; the 54-state network: 1 only.

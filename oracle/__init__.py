@@ -28,6 +28,7 @@ _lib = None
 NCNT = 10
 CNT_NAMES = ("step", "accept", "reject", "newton_fail", "lu", "lu_fail", "newton", "f", "jac", "es")
 MECH_VDP, MECH_LINEAR, MECH_KINETICS = 0, 1, 2
+FD_JACOBIAN = 256  # ORA_FD_JACOBIAN: eval_jacob by generic/fd_jacob.c
 
 
 def build(force: bool = False) -> str:
@@ -125,16 +126,18 @@ class Mechanism:
 
 
 def integrate(mech: Mechanism, t, t_end, pr, y_colmajor, atol=1e-10, rtol=1e-6,
-              const_time_step=False, nthreads=0):
+              const_time_step=False, nthreads=0, fd_jacobian=False):
     """One intDriver call (solver_generic.c:28): y is column-major (n, num),
-    updated copy returned with codes and counters."""
+    updated copy returned with codes and counters.  fd_jacobian: the Jacobian
+    by generic/fd_jacob.c instead of the analytic callback."""
     y = np.array(y_colmajor, dtype=np.float64, order="C", copy=True)
     n, num = y.shape
     assert n == mech.n
     pr = np.ascontiguousarray(pr, dtype=np.float64)
     codes = np.zeros(num, dtype=np.int32)
     cnt = np.zeros((num, NCNT), dtype=np.int64)
-    lib().ora_batch(mech.kind, mech.ctx_ptr, n, atol, rtol, int(const_time_step), num, t, t_end,
+    kind = mech.kind | (FD_JACOBIAN if fd_jacobian else 0)
+    lib().ora_batch(kind, mech.ctx_ptr, n, atol, rtol, int(const_time_step), num, t, t_end,
                     _dp(pr), _dp(y), _ip(codes),
                     cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(nthreads))
     return y, codes, cnt

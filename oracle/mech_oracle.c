@@ -278,6 +278,8 @@ static void make_problem(ora_problem *p, int mech, const void *ctx, int n, doubl
     p->rtol = rtol;
     p->const_time_step = cts;
     p->ctx = ctx;
+    p->fd_jac = (mech & ORA_FD_JACOBIAN) != 0;
+    mech &= ~ORA_FD_JACOBIAN;
     switch (mech) {
         case 0: p->dydt = ora_vdp_dydt; p->jac = ora_vdp_jac; break;
         case 1: p->dydt = ora_linear_dydt; p->jac = ora_linear_jac; break;
@@ -303,25 +305,30 @@ void ora_eval(int mech, const void *ctx, int n, int what, double t, double pr, c
 }
 
 /* generic/fd_jacob.c:24-127 with FD_ORD == 1 */
-void ora_fd_jacob(int mech, const void *ctx, int n, double atol, double rtol, double t, double pr,
-                  const double *cy, double *jac) {
-    ora_problem p;
-    make_problem(&p, mech, ctx, n, atol, rtol, 0);
+void ora_problem_fd_jacob(const ora_problem *p, double t, double pr, const double *cy, double *jac) {
+    const int n = p->n;
     double y[n], dy[n], ewt[n], ftemp[n];
     memcpy(y, cy, n * sizeof(double));
-    p.dydt(t, pr, y, dy, p.ctx);
-    for (int i = 0; i < n; ++i) ewt[i] = atol + (rtol * fabs(y[i]));
+    p->dydt(t, pr, y, dy, p->ctx);
+    for (int i = 0; i < n; ++i) ewt[i] = p->atol + (p->rtol * fabs(y[i]));
     double srur = sqrt(DBL_EPSILON);
     double sum = 0.0;
     for (int i = 0; i < n; ++i) sum += (ewt[i] * dy[i]) * (ewt[i] * dy[i]);
     double fac = sqrt(sum / ((double)n));
-    double r0 = 1000.0 * rtol * DBL_EPSILON * ((double)n) * fac;
+    double r0 = 1000.0 * p->rtol * DBL_EPSILON * ((double)n) * fac;
     for (int j = 0; j < n; ++j) {
         double yj = y[j];
         double r = fmax(srur * fabs(yj), r0 / ewt[j]);
         y[j] = yj + r;
-        p.dydt(t, pr, y, ftemp, p.ctx);
+        p->dydt(t, pr, y, ftemp, p->ctx);
         for (int i = 0; i < n; ++i) jac[i + n * j] = (ftemp[i] - dy[i]) / r;
         y[j] = yj;
     }
+}
+
+void ora_fd_jacob(int mech, const void *ctx, int n, double atol, double rtol, double t, double pr,
+                  const double *cy, double *jac) {
+    ora_problem p;
+    make_problem(&p, mech, ctx, n, atol, rtol, 0);
+    ora_problem_fd_jacob(&p, t, pr, cy, jac);
 }

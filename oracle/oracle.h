@@ -39,7 +39,12 @@ typedef struct {
     ora_dydt_fn dydt;
     ora_jac_fn jac;
     const void *ctx;
+    int fd_jac;            /* 1: eval_jacob is generic/fd_jacob.c over dydt */
 } ora_problem;
+
+/* OR-ed into a batch driver's mechanism kind: use the finite-difference
+ * Jacobian (generic/fd_jacob.c) instead of the analytic one. */
+#define ORA_FD_JACOBIAN 256
 
 /* Per-condition event counters (algorithmic work accounting). */
 enum {
@@ -124,6 +129,7 @@ void ora_eval(int mech, const void *ctx, int n, int what, double t, double pr, c
 /* Finite-difference Jacobian, generic/fd_jacob.c:24-127 (FD_ORD 1). */
 void ora_fd_jacob(int mech, const void *ctx, int n, double atol, double rtol, double t, double pr,
                   const double *y, double *jac);
+void ora_problem_fd_jacob(const ora_problem *p, double t, double pr, const double *cy, double *jac);
 
 #ifdef __cplusplus
 }

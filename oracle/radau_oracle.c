@@ -428,7 +428,8 @@ int ora_integrate(const ora_problem *p, double t_start, double t_end, double pr,
         }
         if (!SkipLU) {
             if (!SkipJac) {
-                p->jac(t, pr, y, A, p->ctx);
+                if (p->fd_jac) ora_problem_fd_jacob(p, t, pr, y, A);
+                else p->jac(t, pr, y, A, p->ctx);
                 cnt[ORA_CNT_JAC]++;
             }
             RK_Decomp(n, H, E1, E2, A, ipiv1, ipiv2, &info);

@@ -401,3 +401,39 @@ def test_launch_shapes_agree_and_autotune(session, net14, tmp_path, system, prec
     finally:
         acc.set_launch_shape(n, precision, shapes[1])
         acc.set_precision(acc.PRECISION_FP64)
+
+
+@pytest.mark.parametrize("system,precision", [("vdp", 0), ("k14", 0), ("k14", 1)])
+def test_fd_jacobian_parity(session, oracle_lib, net14, system, precision):
+    """The finite-difference Jacobian (generic/fd_jacob.c) on the device against
+    the oracle integrating with the same finite-difference Jacobian."""
+    if system == "vdp":
+        num, steps, dt = 1024, 20, 1e-3
+        y0, pr = M.van_der_pol_conditions(num, seed=71)
+        _vdp_setup(num)
+        om = oracle_lib.Mechanism.van_der_pol()
+    else:
+        num, steps, dt = 1024, 5, 1e-6
+        y0, pr = M.initial_conditions(net14, num, seed=72)
+        _kin_setup(net14, num)
+        om = oracle_lib.Mechanism.kinetics(net14)
+    acc.set_precision(precision)
+    acc.set_jacobian(acc.JACOBIAN_FINITE_DIFFERENCE)
+    try:
+        y = y0.copy()
+        acc.integrate(num, 0.0, steps * dt, dt, y, pr)
+        codes = acc.get_status(num)
+        cnt = acc.get_counters(num).astype(np.int64)
+    finally:
+        acc.set_jacobian(acc.JACOBIAN_ANALYTIC)
+        acc.set_precision(acc.PRECISION_FP64)
+    yo, co, cnt_o = oracle_lib.integrate_steps(om, y0, pr, dt, steps, nthreads=8, fd_jacobian=True)
+    assert (codes == co).all()
+    same = (cnt == cnt_o).all(1).mean()
+    we = weighted_err(y, yo)
+    print("%s precision %d FD Jacobian: max weighted err vs oracle %.3g; identical counters %.3f"
+          % (system, precision, we.max(), same))
+    if precision == 0:
+        assert we.max() <= 10.0
+    else:
+        arbitrate("%s FD mixed" % system, y, yo, oracle_lib, om, y0, pr, dt, steps)
