@@ -42,7 +42,7 @@ PROFILES = os.path.join(ROOT, "profiles")
 # and precision (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated): the profile
 # summaries under profiles/ carry the numbers and the commands that made them
 TRAFFIC_PROFILES = {("c2", "mixed"): "k_radau_kinetics54_mixed_131072.md",
-                    ("c3", "mixed"): "k_radau_kinetics54_mixed_131072.md"}
+                    ("c3", "mixed"): "k_radau_kinetics54_mixed_learned_131072.md"}
 
 
 def measured_traffic(config, precision):
