@@ -94,12 +94,25 @@ __global__ void k_mlp_gelu_back(size_t n, float* __restrict__ dA, const float* _
 }
 
 // out[j] = sum_b X[b][j] (deterministic: one thread per column).
-__global__ void k_mlp_colsum(int rows, int cols, const float* __restrict__ X, float* __restrict__ out) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= cols) return;
+// Column sums of a rows x cols row-major matrix (bias gradients): 16 columns
+// per 256-thread block, 16 row phases per column summed through LDS in a fixed
+// order (deterministic).
+__global__ __launch_bounds__(256) void k_mlp_colsum(int rows, int cols, const float* __restrict__ X,
+                                                    float* __restrict__ out) {
+  __shared__ float part[16][17];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int j = blockIdx.x * 16 + tx;
   float s = 0.0f;
-  for (int b = 0; b < rows; ++b) s += X[(size_t)b * cols + j];
-  out[j] = s;
+  if (j < cols)
+    for (int b = ty; b < rows; b += 16) s += X[(size_t)b * cols + j];
+  part[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && j < cols) {
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += part[k][tx];
+    out[j] = t;
+  }
 }
 
 __global__ void k_mlp_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -267,17 +280,17 @@ bool loss_grad(int B, double* sse) {
         *gW3 = G.g + G.off_w3, *gb3 = G.g + G.off_b3;
   // layer 3
   gemm(H, G.outp, B, G.A2, 1, H, G.dY, G.outp, 1, gW3, G.outp, 0);
-  hipLaunchKernelGGL(k_mlp_colsum, dim3(blocks_for(G.outp)), dim3(256), 0, 0, B, G.outp, G.dY, gb3);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((G.outp + 15) / 16), dim3(256), 0, 0, B, G.outp, G.dY, gb3);
   gemm(B, H, G.outp, G.dY, G.outp, 1, W3, 1, G.outp, G.dA2, H, 0);
   hipLaunchKernelGGL(k_mlp_gelu_back, dim3(blocks_for((size_t)B * H)), dim3(256), 0, 0, (size_t)B * H, G.dA2, G.Z2);
   // layer 2
   gemm(H, H, B, G.A1, 1, H, G.dA2, H, 1, gW2, H, 0);
-  hipLaunchKernelGGL(k_mlp_colsum, dim3(blocks_for(H)), dim3(256), 0, 0, B, H, G.dA2, gb2);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((H + 15) / 16), dim3(256), 0, 0, B, H, G.dA2, gb2);
   gemm(B, H, H, G.dA2, H, 1, W2, 1, H, G.dA1, H, 0);
   hipLaunchKernelGGL(k_mlp_gelu_back, dim3(blocks_for((size_t)B * H)), dim3(256), 0, 0, (size_t)B * H, G.dA1, G.Z1);
   // layer 1
   gemm(G.inp, H, B, G.X, 1, G.inp, G.dA1, H, 1, gW1, H, 0);
-  hipLaunchKernelGGL(k_mlp_colsum, dim3(blocks_for(H)), dim3(256), 0, 0, B, H, G.dA1, gb1);
+  hipLaunchKernelGGL(k_mlp_colsum, dim3((H + 15) / 16), dim3(256), 0, 0, B, H, G.dA1, gb1);
   if (!ok(hipGetLastError(), "mlp backward")) return false;
   std::vector<double> part(nb);
   if (!ok(hipMemcpy(part.data(), G.loss, nb * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy")) return false;
