@@ -132,13 +132,26 @@ def cpu_baseline(cfg, mech, y_start, pr, dt, steps, target_cpu_s=160.0):
     num = int(min(y_start.shape[1], max(cores * 4, target_cpu_s / max(per_cond_step * steps, 1e-9))))
     num = max(cores, num - num % cores)
     t0 = time.perf_counter()
-    _, codes, _ = oracle.integrate_steps(om, np.ascontiguousarray(y_start[:, :num]), pr[:num], dt, steps,
-                                         nthreads=cores)
+    y_cpu, codes, _ = oracle.integrate_steps(om, np.ascontiguousarray(y_start[:, :num]), pr[:num], dt, steps,
+                                             nthreads=cores)
     wall = time.perf_counter() - t0
     return {"value": num * steps / wall, "unit": "conditions*steps/s", "cores": cores, "kind": "port",
             "sample": "CPU oracle (C/OpenMP restatement of radau2a.c, plain FP64), first %d conditions "
                       "from the post-warmup states x %d global steps of dt=%g, %d OpenMP threads, %.2f s wall, "
-                      "%d nonzero codes" % (num, steps, dt, cores, wall, int((codes != 0).sum()))}
+                      "%d nonzero codes" % (num, steps, dt, cores, wall, int((codes != 0).sum()))}, y_cpu
+
+
+def errors_vs_oracle(y_gpu, y_cpu):
+    """Max weighted error (atol 1e-10, rtol 1e-6) and max relative error (over
+    |y| > atol) of GPU final states against the FP64 CPU oracle's, on the
+    oracle's conditions (the first columns)."""
+    m = y_cpu.shape[1]
+    g = y_gpu[:, :m]
+    w = np.abs(g - y_cpu) / (1e-10 + 1e-6 * np.abs(y_cpu))
+    big = np.abs(y_cpu) > 1e-10
+    rel = np.abs(g - y_cpu)[big] / np.abs(y_cpu)[big]
+    return {"conditions": m, "max_weighted_err": float(w.max()),
+            "frac_within_10x": float((w.max(0) <= 10.0).mean()), "max_rel_err": float(rel.max()) if rel.size else 0.0}
 
 
 def run_hetero(args):
@@ -445,8 +458,14 @@ def main():
     traffic = measured_traffic(args.config, precision)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(cfg, mech, main_leg["y_warm"], pr, dt, args.steps)
+        cpu, y_cpu = cpu_baseline(cfg, mech, main_leg["y_warm"], pr, dt, args.steps)
         log("cpu baseline: %s" % json.dumps(cpu))
+        # every GPU leg against the FP64 oracle on the oracle's conditions
+        legs_err = {"main": errors_vs_oracle(main_leg["y"], y_cpu)}
+        if compare:
+            for k, leg in legs.items():
+                legs_err[k] = errors_vs_oracle(leg["y"], y_cpu)
+        log("errors vs the FP64 CPU oracle: %s" % json.dumps(legs_err))
     wall = main_leg["wall"]
     method = ("mixed-precision Radau IIA-5 (FP32 iteration-matrix LU and Newton solves, FP64 residuals, "
               "error estimate, step control, state)" if precision == "mixed" else "plain FP64 Radau IIA-5")
@@ -458,6 +477,8 @@ def main():
     config["schedule"] = args.schedule
     if training:
         config["training"] = training
+    if cpu:
+        config["errors_vs_fp64_oracle"] = legs_err
     if compare:
         config.update(compare)
     out = {
