@@ -437,3 +437,26 @@ def test_fd_jacobian_parity(session, oracle_lib, net14, system, precision):
         assert we.max() <= 10.0
     else:
         arbitrate("%s FD mixed" % system, y, yo, oracle_lib, om, y0, pr, dt, steps)
+
+
+def test_negative_stepsize_and_batches_beyond_capacity(session, oracle_lib):
+    """solver_interface.cu conventions: stepsize < 0 means one global step of
+    t_end - t_start (:164); NUM above the initialized capacity is integrated in
+    batches of that capacity (:174-206)."""
+    num, T = 1000, 2e-3
+    y0, mu = M.van_der_pol_conditions(num, seed=81)
+    acc.select_mechanism(acc.MECH_VAN_DER_POL)
+    acc.set_tolerances(ATOL, RTOL)
+    acc.set_const_time_step(False)
+    acc.set_abort_on_error(False)
+    acc.initialize(256, 0)  # capacity 256 < NUM
+    ya = y0.copy()
+    acc.integrate(num, 0.0, T, -1.0, ya, mu)
+    assert (acc.get_status(num) == 0).all()
+    yb = y0.copy()
+    acc.integrate(num, 0.0, T, T, yb, mu)
+    assert np.array_equal(ya, yb)
+    yo, co, _ = oracle_lib.integrate_steps(oracle_lib.Mechanism.van_der_pol(), y0, mu, T, 1, nthreads=8)
+    we = weighted_err(ya, yo)
+    print("batched (4 x 256) single-step run: max weighted err vs oracle %.3g" % we.max())
+    assert (co == 0).all() and we.max() <= 10.0
