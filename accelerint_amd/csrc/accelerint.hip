@@ -1357,14 +1357,7 @@ int accelerInt_integrate_device_hetero(int NUM14, double* y14_dev, const double*
   if (!(t_start + kRoundoff < t_end) || NUM14 + NUM54 == 0) return 0;
   hipStream_t s = stream ? (hipStream_t)stream : S.stream;
   g_launch_msg.clear();
-  S.P.mlp = MlpView{};
-  if (S.learned) {
-    S.P.mlp = mlp_view();
-    if (!S.P.mlp.on || S.P.mlp.n != 54) {
-      set_err("learned start is on but no 54-state network is loaded");
-      return -1;
-    }
-  }
+  S.P.mlp = MlpView{};  // the heterogeneous kernel uses the zero Newton start
   hipError_t e = S.precision == ACCELERINT_PRECISION_MIXED
                      ? launch_hetero<float>(NUM14, var14_dev, y14_dev, codes14_dev, NUM54, var54_dev, y54_dev,
                                             codes54_dev, t_start, t_end, step, s)

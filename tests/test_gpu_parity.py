@@ -460,3 +460,47 @@ def test_negative_stepsize_and_batches_beyond_capacity(session, oracle_lib):
     we = weighted_err(ya, yo)
     print("batched (4 x 256) single-step run: max weighted err vs oracle %.3g" % we.max())
     assert (co == 0).all() and we.max() <= 10.0
+
+
+@pytest.mark.parametrize("n,num", [(14, 1001), (54, 257)])
+def test_ragged_batch_sizes(session, oracle_lib, net14, net54, n, num):
+    """Batch sizes that do not fill the last wave (14 states: four 16-lane
+    groups per wave; 54 states: four waves per block) against the oracle."""
+    mech = net14 if n == 14 else net54
+    steps, dt = 2, 1e-6
+    y0, P = M.initial_conditions(mech, num, seed=90 + n, T_range=(800.0, 2200.0) if n == 54 else (1000.0, 1600.0))
+    _kin_setup(mech, num)
+    y = y0.copy()
+    acc.integrate(num, 0.0, steps * dt, dt, y, P)
+    codes = acc.get_status(num)
+    yo, co, _ = oracle_lib.integrate_steps(oracle_lib.Mechanism.kinetics(mech), y0, P, dt, steps, nthreads=8)
+    assert (codes == co).all()
+    we = weighted_err(y, yo)
+    print("%d-state, %d conditions: max weighted err %.3g" % (n, num, we.max()))
+    assert we.max() <= 10.0
+
+
+def test_empty_batches(session, net14, net54):
+    """Zero conditions: no launch, no error; a heterogeneous launch with one side
+    empty integrates the other side like a single-network launch."""
+    from hipmem import DeviceArray
+    acc.set_kinetics(net14)
+    acc.set_kinetics(net54)
+    acc.set_tolerances(ATOL, RTOL)
+    acc.set_abort_on_error(False)
+    acc.initialize(256, 0)
+    y = np.zeros((54, 0))
+    acc.integrate(0, 0.0, 1e-6, 1e-6, y, np.zeros(0))
+    y54, P54 = M.initial_conditions(net54, 64, seed=93, T_range=(800.0, 2200.0))
+    d54, p54 = DeviceArray(y54.T), DeviceArray(P54)
+    acc.integrate_device(0, 0.0, 1e-6, 1e-6, d54.addr, p54.addr)
+    acc.synchronize()
+    assert np.array_equal(d54.get(), np.ascontiguousarray(y54.T))
+    acc.integrate_device_hetero(0, 0, 0, 0, 64, d54.addr, p54.addr, 0, 0.0, 1e-6, 1e-6)
+    acc.synchronize()
+    ya = d54.get()
+    db = DeviceArray(y54.T)
+    acc.integrate_device(64, 0.0, 1e-6, 1e-6, db.addr, p54.addr)
+    acc.synchronize()
+    assert np.isfinite(ya).all() and not np.array_equal(ya, np.ascontiguousarray(y54.T))
+    assert np.array_equal(ya, db.get())
