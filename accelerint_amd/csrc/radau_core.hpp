@@ -886,6 +886,12 @@ __device__ __forceinline__ void rsolve_cplx(const double (&ar)[N], const double 
 // new condition or a new global step as soon as its current one ends.
 enum : int { RADAU_CONTINUE = -1 };
 
+// pow as a real call.  The step controller calls it at five places where all
+// of the integrator's state is live; inlined, the library routine's many
+// temporaries push the register allocation into spills around every site.
+// Out of line it returns the same values at the cost of a call.
+__device__ __attribute__((noinline)) inline double step_pow(double x, double y) { return pow(x, y); }
+
 template <class Mech, int G, int R, class MT = double>
 struct Radau {
   static constexpr int N = Mech::N;
@@ -1201,7 +1207,7 @@ struct Radau {
     double Fac = 0.5;
     int NewtonIter = 0;
     double Theta = 0.0;
-    NewtonRate = pow(fmax(NewtonRate, kRoundoff), 0.8);
+    NewtonRate = step_pow(fmax(NewtonRate, kRoundoff), 0.8);
 
 #pragma unroll 1
     for (; NewtonIter < kNewtonMaxit; NewtonIter++) {
@@ -1276,10 +1282,10 @@ struct Radau {
           break;
         }
         const double NewtonPredictedErr =
-            (NewtonIncrement * pow(Theta, (double)(kNewtonMaxit - NewtonIter - 1))) / (1.0 - Theta);
+            (NewtonIncrement * step_pow(Theta, (double)(kNewtonMaxit - NewtonIter - 1))) / (1.0 - Theta);
         if (NewtonPredictedErr >= kNewtonTol) {
           const double Qnewton = fmin(10.0, NewtonPredictedErr / kNewtonTol);
-          Fac = 0.8 * pow(Qnewton, -1.0 / ((double)(kNewtonMaxit - NewtonIter)));
+          Fac = 0.8 * step_pow(Qnewton, -1.0 / ((double)(kNewtonMaxit - NewtonIter)));
           break;
         }
       }
@@ -1349,14 +1355,14 @@ struct Radau {
         for (int q = 0; q < R; ++q) Tm[q] = F1[q] + F2[q];
       }
     }
-    Fac = pow(Err, (-1.0 / rkELO)) * (1.0 + 2 * kNewtonMaxit) / (NewtonIter + 1.0 + 2 * kNewtonMaxit);
+    Fac = step_pow(Err, (-1.0 / rkELO)) * (1.0 + 2 * kNewtonMaxit) / (NewtonIter + 1.0 + 2 * kNewtonMaxit);
     Fac = fmin(kFacMax, fmax(kFacMin, Fac));
     double Hnew = Fac * H;
     if (Err < 1.0) {
       cnt[CNT_ACCEPT]++;
       if (FirstStep && P.log.cap > 0) log_first_step(P.log, pr);
       if (!FirstStep) {  // Gustafsson, :676-685
-        double FacGus = kFacSafe * (H / Hacc) * pow(Err * Err / ErrOld, -0.25);
+        double FacGus = kFacSafe * (H / Hacc) * step_pow(Err * Err / ErrOld, -0.25);
         FacGus = fmin(kFacMax, fmax(kFacMin, FacGus));
         Fac = fmin(Fac, FacGus);
         Hnew = Fac * H;
