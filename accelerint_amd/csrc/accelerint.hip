@@ -130,11 +130,12 @@ __device__ __forceinline__ float work_estimate(const uint32_t (&cnt)[NCNT]) {
 // pressure to runs without it).
 template <class Mech, int G, int R, class MT, bool LS = false>
 __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, double t_end, double step,
-                                          const SolverParams& P, typename Mech::Ctx& ctx) {
+                                          const SolverParams& P, typename Mech::Ctx& ctx, double* zs = nullptr) {
   constexpr int N = Mech::N;
   const int ln = Grp<G>::lane();
   const int num = qa.num_dev ? *qa.num_dev : qa.num;
   Radau<Mech, G, R, MT> st;
+  st.zs = zs;
   uint32_t cnt[NCNT];
   double prc = 0.0, t_next = t_end;
   int k = 0;
@@ -204,12 +205,14 @@ __device__ __forceinline__ void run_queue(const Queue& qa, double t_start, doubl
 
 template <class Mech, int G, int R, int BLOCK, int WPE, class MT, bool LS = false>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_radau(
-    Queue qa, double t_start, double t_end, double step, SolverParams P, typename Mech::Params mp) {
+    Queue qa, double t_start, double t_end, double step, SolverParams P, typename Mech::Params mp, int start_off) {
   extern __shared__ double lds_dyn[];
   const int g = threadIdx.x / G;
   Mech::block_init(mp, lds_dyn);
   typename Mech::Ctx ctx(mp, g, lds_dyn, Radau<Mech, G, R, MT>::kNeedsScratch);
-  run_queue<Mech, G, R, MT, LS>(qa, t_start, t_end, step, P, ctx);
+  // learned start: the group's predicted stages after the mechanism's LDS
+  double* zs = LS ? lds_dyn + start_off + (size_t)g * 3 * G * R : nullptr;
+  run_queue<Mech, G, R, MT, LS>(qa, t_start, t_end, step, P, ctx, zs);
 }
 
 // Heterogeneous batch in one persistent launch: a 54-state and a 14-state
@@ -349,7 +352,9 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
                         const typename Mech::Params& mp, hipStream_t s, float* cost = nullptr) {
   constexpr int GPB = BLOCK / G;
   auto kern = k_radau<Mech, G, R, BLOCK, WPE, MT, LS>;
-  const size_t lds = Mech::lds_bytes(mp, GPB, Radau<Mech, G, R, MT>::kNeedsScratch);
+  const size_t lds_mech = Mech::lds_bytes(mp, GPB, Radau<Mech, G, R, MT>::kNeedsScratch);
+  // learned start: 3 stages x R rows x G lanes of doubles per group after the mechanism's share
+  const size_t lds = lds_mech + (LS ? (size_t)GPB * 3 * G * R * sizeof(double) : 0);
   static int resident = 0;     // blocks per device that fit at once, per instantiation
   static size_t lds_ok = 0;    // LDS size the residency was computed for
   if (resident == 0 || lds_ok != lds) {
@@ -381,7 +386,8 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
   if (blocks > resident) blocks = resident;
   hipLaunchKernelGGL(k_reset_work, dim3(1), dim3(1), 0, s, work, esc);
   const Queue qa{num, pr, y, codes, cnt, work, list, num_dev, esc, cost};
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, qa, t_start, t_end, step, P, mp);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(BLOCK), lds, s, qa, t_start, t_end, step, P, mp,
+                     (int)(lds_mech / sizeof(double)));
   return hipGetLastError();
 }
 

@@ -932,15 +932,22 @@ struct Radau {
   double A[R][N];  // Jacobian (only live across attempts when !Mech::kJacInLds)
   // predicted Newton start of the current global step's first step; dropped
   // (zero start, the reference's rule) after any rejection of that step
-  double Zs1[R], Zs2[R], Zs3[R];
+  // kept in the group's LDS slice (zs[s*G*R + q*G + lane], s = stage), not in
+  // registers: it is read once, at the start of the first step
+  double* zs = nullptr;
   bool UseStart;
   // set when a predicted start was discarded (by the first-iteration test or
   // a Newton failure); the caller stops predicting for this condition
   bool StartFailed;
 
   __device__ __forceinline__ void set_start(const double (&z1)[R], const double (&z2)[R], const double (&z3)[R]) {
+    const int ln = Grp<G>::lane();
 #pragma unroll
-    for (int q = 0; q < R; ++q) { Zs1[q] = z1[q]; Zs2[q] = z2[q]; Zs3[q] = z3[q]; }
+    for (int q = 0; q < R; ++q) {
+      zs[q * G + ln] = z1[q];
+      zs[(R + q) * G + ln] = z2[q];
+      zs[(2 * R + q) * G + ln] = z3[q];
+    }
     UseStart = true;
   }
 
@@ -1187,9 +1194,9 @@ struct Radau {
     if (FirstStep) {
 #pragma unroll
       for (int q = 0; q < R; ++q) {
-        Z1[q] = UseStart ? Zs1[q] : 0.0;
-        Z2[q] = UseStart ? Zs2[q] : 0.0;
-        Z3[q] = UseStart ? Zs3[q] : 0.0;
+        Z1[q] = UseStart ? zs[q * G + ln] : 0.0;
+        Z2[q] = UseStart ? zs[(R + q) * G + ln] : 0.0;
+        Z3[q] = UseStart ? zs[(2 * R + q) * G + ln] : 0.0;
       }
     } else {
       // RK_Interpolate, :302-314 (third stage keeps the reference's x2 factor)
