@@ -107,7 +107,7 @@ def algorithmic_flops(cnt, n, F, Jc):
             + es * (2 * n * n + 8 * n) + f * F + jac * Jc + step * 40 * n)
 
 
-def cpu_baseline(cfg, mech, y_start, pr, dt, steps, target_cpu_s=160.0):
+def cpu_baseline(cfg, mech, y_start, pr, dt, steps, target_cpu_s=80.0):
     """Times the CPU oracle (all host cores, at most 16) on a bounded sample of
     the timed workload: the first conditions, starting from the same
     post-warmup states the GPU's timed launch starts from, over the same
@@ -131,14 +131,24 @@ def cpu_baseline(cfg, mech, y_start, pr, dt, steps, target_cpu_s=160.0):
     per_cond_step = (time.perf_counter() - t0) * cores / num  # CPU-seconds per condition-step
     num = int(min(y_start.shape[1], max(cores * 4, target_cpu_s / max(per_cond_step * steps, 1e-9))))
     num = max(cores, num - num % cores)
+    ys, ps = np.ascontiguousarray(y_start[:, :num]), pr[:num]
+    walls = []
+    for _ in range(3):  # median of repeats, as the reference's runner.py:74 times its executables
+        t0 = time.perf_counter()
+        y_cpu, codes, _ = oracle.integrate_steps(om, ys, ps, dt, steps, nthreads=cores)
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
+    # one thread on a sixteenth of the sample
+    m1 = max(1, num // 16)
     t0 = time.perf_counter()
-    y_cpu, codes, _ = oracle.integrate_steps(om, np.ascontiguousarray(y_start[:, :num]), pr[:num], dt, steps,
-                                             nthreads=cores)
-    wall = time.perf_counter() - t0
+    oracle.integrate_steps(om, np.ascontiguousarray(ys[:, :m1]), ps[:m1], dt, steps, nthreads=1)
+    single = m1 * steps / (time.perf_counter() - t0)
     return {"value": num * steps / wall, "unit": "conditions*steps/s", "cores": cores, "kind": "port",
+            "single_thread_value": single,
             "sample": "CPU oracle (C/OpenMP restatement of radau2a.c, plain FP64), first %d conditions "
-                      "from the post-warmup states x %d global steps of dt=%g, %d OpenMP threads, %.2f s wall, "
-                      "%d nonzero codes" % (num, steps, dt, cores, wall, int((codes != 0).sum()))}, y_cpu
+                      "from the post-warmup states x %d global steps of dt=%g, %d OpenMP threads, median of 3 "
+                      "runs %.2f s wall, %d nonzero codes; 1 thread on the first %d: %.4g conditions*steps/s"
+                      % (num, steps, dt, cores, wall, int((codes != 0).sum()), m1, single)}, y_cpu
 
 
 def errors_vs_oracle(y_gpu, y_cpu):
