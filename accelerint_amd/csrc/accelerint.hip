@@ -1301,6 +1301,17 @@ void accelerInt_integrate(const int NUM, const double t_start, const double t_en
   const int n = S.n;
   int num_solved = 0;
   if ((int)S.h_codes.size() < NUM) S.h_codes.assign(NUM, 0);
+  // batches of different conditions: one batch's measured work says nothing
+  // about the next one's, so batched calls take conditions in index order
+  const int schedule = S.schedule;
+  if (NUM > S.padded) {
+    S.schedule = ACCELERINT_SCHEDULE_INDEX;
+    S.cost_num = S.order_num = 0;
+  }
+  struct RestoreSchedule {
+    int mode;
+    ~RestoreSchedule() { S.schedule = mode; }
+  } restore{schedule};
   std::vector<int> all_codes(NUM, 0);
   std::vector<double> stage;
   while (num_solved < NUM) {

@@ -167,9 +167,11 @@ int accelerInt_set_abort_on_error(int on);
  * non-zero codes, or -1 if no call has been made. */
 int accelerInt_get_status(int *codes, int num);
 
-/* Per-condition event counters accumulated since initialize:
- * out[c*10 + k], k = step, accept, reject, newton_fail, lu, lu_fail, newton,
- * f, jac, es.  Returns 0 on success. */
+/* Per-condition event counters accumulated since initialize or the last
+ * reset: out[c*10 + k], k = step, accept, reject, newton_fail, lu, lu_fail,
+ * newton, f, jac, es; num at most the capacity given to initialize (a call
+ * with more conditions runs in batches of that capacity, each adding into the
+ * slots of its own first conditions).  Returns 0 on success. */
 int accelerInt_get_counters(uint32_t *out, int num);
 int accelerInt_reset_counters(void);
 
