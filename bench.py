@@ -286,6 +286,9 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain-FP64 comparison leg of a mixed-precision run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the K steps as K replays of a hipGraph of one global step (from the same "
+                         "post-warmup states; compared with the single K-step persistent launch)")
     ap.add_argument("--schedule", choices=["longest-first", "index"], default="longest-first",
                     help="queue order of the persistent kernel (results do not depend on it)")
     ap.add_argument("--train-samples", type=int, default=2000000,
@@ -454,6 +457,29 @@ def main():
             compare["worst8_vs_cpu_oracle"] = {"main": werr(main_leg["y"][:, worst], yo),
                                                "fp64": werr(ref["y"][:, worst], yo)}
         log("compare: %s" % json.dumps(compare))
+    if args.graph:
+        # one global step captured as a graph, replayed K times from the same states
+        acc.set_precision(acc.PRECISION_MIXED if precision == "mixed" else acc.PRECISION_FP64)
+        acc.set_learned_start(learned_on)
+        y_g = torch.from_numpy(np.ascontiguousarray(main_leg["y_warm"].T)).to(dev)
+        t_g = args.warmup * dt
+        torch.cuda.synchronize()
+        acc.graph_capture(num, t_g, t_g + dt, dt, y_g.data_ptr(), pr_dev.data_ptr(), codes_dev.data_ptr(), sp)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(stream)
+        acc.graph_launch(args.steps, sp)
+        end.record(stream)
+        torch.cuda.synchronize()
+        wall_g = time.perf_counter() - t0
+        acc.graph_release()
+        acc.set_learned_start(False)
+        yg = y_g.cpu().numpy().T
+        dg = float((np.abs(yg - main_leg["y"]) / (1e-10 + 1e-6 * np.abs(main_leg["y"]))).max())
+        graph = {"value": num * args.steps / wall_g, "ms": start.elapsed_time(end),
+                 "vs_persistent_launch": main_leg["wall"] / wall_g, "max_weighted_diff_vs_persistent": dg}
+        log("graph replays (%d x one global step): %s" % (args.steps, json.dumps(graph)))
+        compare = dict(compare or {}, graph_replays=graph)
 
     if args.calib_copy_mb:
         # a plain copy of known size, for scaling rocprof FETCH_SIZE/WRITE_SIZE
