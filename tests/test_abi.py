@@ -53,3 +53,33 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(acc, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(acc.AccelerIntError):
         acc.lib()
+
+
+@pytest.mark.parametrize("code,text", [(1, "too many consecutive integration steps"),
+                                       (2, "allowed number of integration steps was exceeded"),
+                                       (3, "decreased such that h = t + h"),
+                                       (4, "allowed number of newton iteration steps was exceeded")])
+def test_check_error_exits_with_the_code(code, text):
+    """check_error (radau2a_props.cu:15-40): the first condition with a non-zero
+    code prints the reference's message and exits with that code (host only)."""
+    import subprocess
+    import sys
+    if not os.path.exists(acc.LIB_PATH):
+        pytest.skip("libaccelerint_hip.so not built")
+    prog = ("import ctypes; L = ctypes.CDLL(%r); c = (ctypes.c_int * 4)(0, 0, %d, 1); L.check_error(4, c); "
+            "print('returned')" % (acc.LIB_PATH, code))
+    r = subprocess.run([sys.executable, "-c", prog], capture_output=True, text=True, timeout=60)
+    assert r.returncode == code
+    assert ("During integration of ODE# 2, " in r.stdout) and (text in r.stdout)
+    assert "returned" not in r.stdout
+
+
+def test_check_error_passes_clean_codes():
+    import subprocess
+    import sys
+    if not os.path.exists(acc.LIB_PATH):
+        pytest.skip("libaccelerint_hip.so not built")
+    prog = ("import ctypes; L = ctypes.CDLL(%r); c = (ctypes.c_int * 3)(0, 0, 0); L.check_error(3, c); "
+            "print('returned')" % acc.LIB_PATH)
+    r = subprocess.run([sys.executable, "-c", prog], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "returned" in r.stdout

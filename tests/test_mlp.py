@@ -211,3 +211,83 @@ def test_learned_start_garbage_network_falls_back(accmod, oracle_lib):
                            "garbage network")
     print("Newton iterations: learned %d, zero start %d" % (out[True][2][:, 6].sum(), out[False][2][:, 6].sum()))
     assert out[True][2][:, 6].sum() > out[False][2][:, 6].sum()
+
+
+@pytest.mark.gpu
+def test_sample_log_into_caller_buffers(accmod):
+    """accelerInt_log_first_steps: one sample per condition and global step in
+    caller-owned device buffers, identical to what the library-owned log
+    (accelerInt_start_log / accelerInt_read_log) records for the same
+    integration; the first global step's samples carry the features of the
+    initial states (1000/T, log Y_k, log p) and an h no larger than the
+    initial 5e-7 (rejections before the first acceptance only shrink it)."""
+    from hipmem import DeviceArray
+    from accelerint_amd import mechanism as M
+    acc = accmod
+    mech = M.network(54)
+    num, steps, dt = 256, 2, 1e-6
+    y0, P = M.initial_conditions(mech, num, seed=95, T_range=(800.0, 2200.0))
+    acc.set_kinetics(mech)
+    acc.set_tolerances(1e-10, 1e-6)
+    acc.set_const_time_step(False)
+    acc.set_abort_on_error(False)
+    acc.set_precision(acc.PRECISION_FP64)
+    acc.initialize(num, 0)
+
+    def rows_sorted(a):
+        return a[np.lexsort(a.T[::-1])]
+
+    try:
+        cap = num * steps + 16
+        xd = DeviceArray(np.zeros((cap, 56), np.float32))
+        zd = DeviceArray(np.zeros((cap, 162), np.float32))
+        cd = DeviceArray(np.zeros(1, np.int32))
+        acc.log_first_steps(xd.addr, zd.addr, cap, cd.addr)
+        y = y0.copy()
+        acc.integrate(num, 0.0, dt, dt, y, P)  # first global step: one sample per condition
+        n1 = int(cd.get()[0])
+        X1 = xd.get()[:n1]
+        acc.integrate(num, dt, steps * dt, dt, y, P)
+        n = int(cd.get()[0])
+        acc.log_first_steps(0, 0, 0, 0)
+        X, Z = xd.get()[:n], zd.get()[:n]
+        assert n1 == num and n == num * steps
+        assert np.isfinite(X).all() and np.isfinite(Z).all()
+        ref = np.concatenate([1000.0 / y0[:1].T, np.log(np.maximum(y0[1:].T, 1e-30)),
+                              np.log(P)[:, None]], axis=1).astype(np.float32)
+        got = X1[:, list(range(54)) + [55]]
+        assert np.allclose(rows_sorted(got), rows_sorted(ref), rtol=1e-5, atol=1e-5)
+        assert (X1[:, 54] <= np.float32(np.log(5e-7)) + 1e-5).all()
+
+        acc.start_log(cap)
+        y2 = y0.copy()
+        acc.integrate(num, 0.0, steps * dt, dt, y2, P)
+        X2, Z2 = acc.read_log(cap, 54)  # also ends the library-owned log
+        assert X2.shape[0] == n
+        XZ, XZ2 = np.concatenate([X, Z], 1), np.concatenate([X2, Z2], 1)
+        assert np.array_equal(rows_sorted(XZ), rows_sorted(XZ2))
+    finally:
+        acc.cleanup()
+
+
+@pytest.mark.gpu
+def test_device_resident_training_matches_host_entry(accmod):
+    """accelerInt_mlp_train on device buffers and accelerInt_mlp_train_host on
+    the same samples from the same initial network give the same parameters."""
+    from hipmem import DeviceArray
+    acc = accmod
+    rng = np.random.default_rng(17)
+    net = mlp_ref.Net(N54)
+    n = 2048
+    x = rng.standard_normal((n, net.inn)).astype(np.float32)
+    t = (0.3 * rng.standard_normal((n, net.out))).astype(np.float32)
+    acc.mlp_init(N54, 9)
+    h_host = acc.mlp_train_host(x, t, 2, 256, 1e-3, 5)
+    p_host = acc.mlp_params()
+    acc.mlp_init(N54, 9)
+    xd, td = DeviceArray(x), DeviceArray(t)
+    h_dev = acc.mlp_train(n, xd.addr, td.addr, 2, 256, 1e-3, 5)
+    p_dev = acc.mlp_params()
+    print("loss per epoch: host %s device %s" % (h_host, h_dev))
+    assert h_dev[-1] < h_dev[0]
+    assert np.array_equal(p_host, p_dev) and np.array_equal(h_host, h_dev)
