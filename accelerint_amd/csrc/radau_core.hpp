@@ -252,6 +252,48 @@ __device__ __forceinline__ double dpp_t(double v) {
 }
 constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
 // (value, key) order of the pivot search: larger value, then smaller key.
+// Cross-row exchanges of a 64-lane wave on the VALU (gfx950
+// v_permlane16_swap / v_permlane32_swap).  Called with a == b == v:
+// swap_row_pairs leaves a = v of rows (0, 0, 2, 2) and b = v of rows
+// (1, 1, 3, 3) in rows 0..3; swap_halves leaves a = v of lanes 0-31 and
+// b = v of lanes 32-63 in both halves.
+__device__ __forceinline__ void swap_row_pairs(int& a, int& b) {
+  const auto r = __builtin_amdgcn_permlane16_swap((unsigned)a, (unsigned)b, false, false);
+  a = (int)r[0];
+  b = (int)r[1];
+}
+__device__ __forceinline__ void swap_halves(int& a, int& b) {
+  const auto r = __builtin_amdgcn_permlane32_swap((unsigned)a, (unsigned)b, false, false);
+  a = (int)r[0];
+  b = (int)r[1];
+}
+__device__ __forceinline__ void swap_row_pairs(float& a, float& b) {
+  int x = __float_as_int(a), y = __float_as_int(b);
+  swap_row_pairs(x, y);
+  a = __int_as_float(x);
+  b = __int_as_float(y);
+}
+__device__ __forceinline__ void swap_halves(float& a, float& b) {
+  int x = __float_as_int(a), y = __float_as_int(b);
+  swap_halves(x, y);
+  a = __int_as_float(x);
+  b = __int_as_float(y);
+}
+__device__ __forceinline__ void swap_row_pairs(double& a, double& b) {
+  int al = __double2loint(a), ah = __double2hiint(a), bl = __double2loint(b), bh = __double2hiint(b);
+  swap_row_pairs(al, bl);
+  swap_row_pairs(ah, bh);
+  a = __hiloint2double(ah, al);
+  b = __hiloint2double(bh, bl);
+}
+__device__ __forceinline__ void swap_halves(double& a, double& b) {
+  int al = __double2loint(a), ah = __double2hiint(a), bl = __double2loint(b), bh = __double2hiint(b);
+  swap_halves(al, bl);
+  swap_halves(ah, bh);
+  a = __hiloint2double(ah, al);
+  b = __hiloint2double(bh, bl);
+}
+
 // Bitwise, not short-circuit, so the exchange stays branch-free.
 template <class T>
 __device__ __forceinline__ void argmax_merge(T& bv, int& bkey, T ov, int ok) {
@@ -273,18 +315,35 @@ __device__ __forceinline__ void group_argmax(T& bv, int& bkey) {
     argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
   } else if constexpr (G == 64) {
-    // DPP within each 16-lane row, then the row winners by two xor exchanges.
-    // (Two other ways to join the rows picked wrong pivots on gfx950 and are
-    // not used: merging the four row winners after reading them into scalar
-    // registers, and DPP row_bcast:15 / row_bcast:31 with row masks.)
+    // DPP within each 16-lane row, then the row winners joined by the VALU
+    // lane swaps (row pairs 0|1 and 2|3, then halves), which replace two
+    // ds_bpermute round trips per exchange.  Every lane merges (lower, upper)
+    // in that order, so all 64 lanes end with the same winner.  (Two other
+    // ways to join the rows picked wrong pivots on gfx950 and are not used:
+    // merging the four row winners after reading them into scalar registers,
+    // and DPP row_bcast:15 / row_bcast:31 with row masks.)
     argmax_merge(bv, bkey, dpp_t<kDppXor1>(bv), dpp_i<kDppXor1>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppXor2>(bv), dpp_i<kDppXor2>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppHalfMirror>(bv), dpp_i<kDppHalfMirror>(bkey));
     argmax_merge(bv, bkey, dpp_t<kDppMirror>(bv), dpp_i<kDppMirror>(bkey));
-    // rows 1 and 3 take the winners of rows 0 and 2, then rows 2 and 3 the
-    // winner of rows 0-1; lane 63 ends with the wave's winner
-    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 16), Grp<G>::xor_i(bkey, 16));
-    argmax_merge(bv, bkey, Grp<G>::xor_(bv, 32), Grp<G>::xor_i(bkey, 32));
+    {
+      T lo = bv, hi = bv;
+      int klo = bkey, khi = bkey;
+      swap_row_pairs(lo, hi);
+      swap_row_pairs(klo, khi);
+      bv = lo;
+      bkey = klo;
+      argmax_merge(bv, bkey, hi, khi);
+    }
+    {
+      T lo = bv, hi = bv;
+      int klo = bkey, khi = bkey;
+      swap_halves(lo, hi);
+      swap_halves(klo, khi);
+      bv = lo;
+      bkey = klo;
+      argmax_merge(bv, bkey, hi, khi);
+    }
   } else {
 #pragma unroll
     for (int m = 1; m < G; m <<= 1) argmax_merge(bv, bkey, Grp<G>::xor_(bv, m), Grp<G>::xor_i(bkey, m));
