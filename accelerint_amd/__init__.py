@@ -53,7 +53,7 @@ EXPORTS = (
     "accelerInt_set_schedule", "accelerInt_get_schedule", "read_initial_conditions",
     "accelerInt_read_initial_conditions", "accelerInt_write_log", "accelerInt_load_kinetics",
     "accelerInt_set_launch_shape", "accelerInt_get_launch_shape", "accelerInt_autotune", "accelerInt_load_tuning",
-    "accelerInt_set_jacobian",
+    "accelerInt_set_jacobian", "accelerInt_factor_test",
 )
 
 
@@ -97,6 +97,7 @@ def lib():
         "accelerInt_synchronize": ([], i),
         "accelerInt_last_error": ([], ctypes.c_char_p),
         "accelerInt_eval_callbacks": ([i, d, v, v, v, v], i),
+        "accelerInt_factor_test": ([i, i, i, v, v, v, v, v, v], i),
         "accelerInt_set_precision": ([i], i),
         "accelerInt_escalated": ([], i),
         "accelerInt_set_schedule": ([i], i),
@@ -490,6 +491,27 @@ def eval_callbacks(NUM: int, t: float, y_host: np.ndarray, var_host: np.ndarray)
                                            dy.ctypes.data, jac.ctypes.data),
            "accelerInt_eval_callbacks")
     return dy, jac.reshape(NUM, n, n).transpose(0, 2, 1).copy()
+
+
+def factor_test(a: np.ndarray, precision: int = PRECISION_FP64):
+    """The integrator's device LU on matrices a (NUM, n, n), real or complex,
+    n in (2, 14, 54); returns (lu, ipiv, info) as LAPACK dgetrf_/zgetrf_ give
+    them per matrix: the combined unit-L / U factors of P a (NUM, n, n), the
+    1-based interchanges (NUM, n) and the first zero pivot (NUM,)."""
+    a = np.asarray(a)
+    num, n, _ = a.shape
+    cplx = np.iscomplexobj(a)
+    are = np.ascontiguousarray(a.real, dtype=np.float64)
+    aim = np.ascontiguousarray(a.imag, dtype=np.float64) if cplx else None
+    lre = np.zeros_like(are)
+    lim = np.zeros_like(are) if cplx else None
+    ipiv = np.zeros((num, n), dtype=np.int32)
+    info = np.zeros(num, dtype=np.int32)
+    _check(lib().accelerInt_factor_test(int(n), int(precision), int(num), are.ctypes.data,
+                                        aim.ctypes.data if cplx else None, lre.ctypes.data,
+                                        lim.ctypes.data if cplx else None, ipiv.ctypes.data, info.ctypes.data),
+           "accelerInt_factor_test")
+    return (lre + 1j * lim if cplx else lre), ipiv, info
 
 
 def synchronize():

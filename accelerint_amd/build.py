@@ -11,13 +11,14 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaccelerint_hip.so")
-SOURCES = [os.path.join(CSRC, "accelerint.hip"), os.path.join(CSRC, "mlp.hip")]
+SOURCES = [os.path.join(CSRC, "accelerint.hip"), os.path.join(CSRC, "mlp.hip"), os.path.join(CSRC, "factor_test.hip")]
 HEADERS = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")] + [
     os.path.join(ROOT, "include", "accelerint.h")]
 
@@ -39,14 +40,34 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False, extra_flags=()) -> str:
+    """Compiles the sources in parallel (one hipcc per file) and links the library."""
     tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = [hipcc(), *HIPCC_FLAGS, *extra_flags, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-           "-o", tmp, *SOURCES]
+    objdir = tempfile.mkdtemp(prefix="accelerint_build_")
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
     t0 = time.time()
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building %s" % LIB)
+    try:
+        jobs = []
+        for src in SOURCES:
+            obj = os.path.join(objdir, os.path.splitext(os.path.basename(src))[0] + ".o")
+            cmd = [hipcc(), *compile_flags, *extra_flags, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                   "-c", "-o", obj, src]
+            jobs.append((src, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                                    text=True)))
+        failed = []
+        for src, obj, proc in jobs:
+            out, _ = proc.communicate()
+            if proc.returncode != 0:
+                sys.stderr.write(out)
+                failed.append(os.path.basename(src))
+        if failed:
+            raise RuntimeError("hipcc failed compiling %s" % ", ".join(failed))
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *[obj for _, obj, _ in jobs]]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed linking %s" % LIB)
+    finally:
+        shutil.rmtree(objdir, ignore_errors=True)
     os.replace(tmp, LIB)
     if verbose:
         print("built %s in %.1fs" % (LIB, time.time() - t0))

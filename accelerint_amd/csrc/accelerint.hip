@@ -391,6 +391,10 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
   return hipGetLastError();
 }
 
+// factor_test.hip: the row LU on caller matrices (orders 2, 14, 54).
+hipError_t launch_factor_test(int n, bool fp32, bool cplx, int num, const double* are, const double* aim,
+                              double* lre, double* lim, int* perm, int* info, hipStream_t s);
+
 }  // namespace acc
 
 using namespace acc;
@@ -1247,6 +1251,7 @@ int accelerInt_num_states(void) { return S.n; }
 
 // solver_interface.cu:152-211
 void accelerInt_initialize(int NUM, int device) {
+  S.err.clear();  // void entry points report failure only through the last error
   device = device < 0 ? 0 : device;
   int num_devices = 0;
   if (hipGetDeviceCount(&num_devices) != hipSuccess) num_devices = 0;
@@ -1297,6 +1302,7 @@ void accelerInt_initialize(int NUM, int device) {
 // the transfers done as one contiguous copy plus a device transposition.
 void accelerInt_integrate(const int NUM, const double t_start, const double t_end,
                           const double stepsize, double* y_host, const double* var_host) {
+  S.err.clear();
   if (!ensure_init()) return;
   const int n = S.n;
   int num_solved = 0;
@@ -1438,6 +1444,69 @@ int accelerInt_eval_callbacks(int NUM, double t, const double* y_host, const dou
   return rc;
 }
 
+int accelerInt_factor_test(int n, int precision, int NUM, const double* a_re, const double* a_im, double* lu_re,
+                           double* lu_im, int* ipiv, int* info) {
+  if (!ensure_init()) return -1;
+  if ((n != 2 && n != 14 && n != 54) || NUM <= 0 || NUM > (1 << 20) || !a_re || !lu_re || !ipiv || !info ||
+      (a_im && !lu_im) || (precision != ACCELERINT_PRECISION_FP64 && precision != ACCELERINT_PRECISION_MIXED)) {
+    set_err("accelerInt_factor_test: bad arguments (n must be 2, 14 or 54; NUM 1..%d)", 1 << 20);
+    return -2;
+  }
+  const bool cplx = a_im != nullptr;
+  const size_t mat = (size_t)NUM * n * n;
+  double *d_are = nullptr, *d_aim = nullptr, *d_lre = nullptr, *d_lim = nullptr;
+  int *d_perm = nullptr, *d_info = nullptr;
+  int rc = 0;
+  std::vector<int> perm((size_t)NUM * n);
+  if (!HIPOK(hipMalloc(&d_are, mat * sizeof(double))) || !HIPOK(hipMalloc(&d_lre, mat * sizeof(double))) ||
+      (cplx && (!HIPOK(hipMalloc(&d_aim, mat * sizeof(double))) || !HIPOK(hipMalloc(&d_lim, mat * sizeof(double))))) ||
+      !HIPOK(hipMalloc(&d_perm, (size_t)NUM * n * sizeof(int))) || !HIPOK(hipMalloc(&d_info, NUM * sizeof(int)))) {
+    rc = -1;
+  } else if (!HIPOK(hipMemcpyAsync(d_are, a_re, mat * sizeof(double), hipMemcpyHostToDevice, S.stream)) ||
+             (cplx && !HIPOK(hipMemcpyAsync(d_aim, a_im, mat * sizeof(double), hipMemcpyHostToDevice, S.stream)))) {
+    rc = -1;
+  } else if (!hip_ok(launch_factor_test(n, precision == ACCELERINT_PRECISION_MIXED, cplx, NUM, d_are, d_aim, d_lre,
+                                        d_lim, d_perm, d_info, S.stream),
+                     "k_factor_test")) {
+    rc = -1;
+  } else if (!HIPOK(hipMemcpyAsync(lu_re, d_lre, mat * sizeof(double), hipMemcpyDeviceToHost, S.stream)) ||
+             (cplx && !HIPOK(hipMemcpyAsync(lu_im, d_lim, mat * sizeof(double), hipMemcpyDeviceToHost, S.stream))) ||
+             !HIPOK(hipMemcpyAsync(perm.data(), d_perm, perm.size() * sizeof(int), hipMemcpyDeviceToHost, S.stream)) ||
+             !HIPOK(hipMemcpyAsync(info, d_info, NUM * sizeof(int), hipMemcpyDeviceToHost, S.stream)) ||
+             !HIPOK(hipStreamSynchronize(S.stream))) {
+    rc = -1;
+  } else {
+    // pivot rows -> LAPACK interchanges: at[p] = original row now at position
+    // p, where[r] = position of original row r; step k swaps positions k and
+    // where[perm[k]]
+    std::vector<int> at(n), where(n);
+    for (int c = 0; c < NUM && rc == 0; ++c) {
+      for (int i = 0; i < n; ++i) at[i] = where[i] = i;
+      for (int k = 0; k < n; ++k) {
+        const int r = perm[(size_t)c * n + k];
+        if (r < 0 || r >= n || where[r] < k) {
+          set_err("accelerInt_factor_test: matrix %d: pivot row %d at step %d is not a remaining row", c, r, k);
+          rc = -3;
+          break;
+        }
+        const int p = where[r], sw = at[k];
+        ipiv[(size_t)c * n + k] = p + 1;
+        at[p] = sw;
+        where[sw] = p;
+        at[k] = r;
+        where[r] = k;
+      }
+    }
+  }
+  hipFree(d_are);
+  hipFree(d_aim);
+  hipFree(d_lre);
+  hipFree(d_lim);
+  hipFree(d_perm);
+  hipFree(d_info);
+  return rc;
+}
+
 int accelerInt_escalated(void) {
   if (S.device < 0 || !S.d_esc) return -1;
   int n = 0;
@@ -1511,6 +1580,7 @@ int accelerInt_synchronize(void) {
 }
 
 void accelerInt_cleanup(void) {
+  S.err.clear();
   if (S.device < 0) return;
   hipStreamSynchronize(S.stream);
   accelerInt_graph_release();

@@ -1,0 +1,97 @@
+// Device LU without integration: the integrator's row-LU routines (lu_real,
+// lu_cplx of radau_core.hpp) run on caller matrices so that their pivot
+// choice, factors and zero-pivot report can be checked one to one against
+// LAPACK dgetrf_/zgetrf_, the routines the reference calls in RK_Decomp
+// (radau2a.c:273,277).  One launcher per order the integrator compiles
+// (2: one system per lane; 14: 16-lane groups; 54: one wave per system).
+#include <hip/hip_runtime.h>
+
+#include "radau_core.hpp"
+
+namespace acc {
+
+// One group per matrix.  a_re/a_im: row-major N x N per matrix (a_im unused
+// for the real LU).  Row r of the factored matrix is written to the row of
+// the LAPACK factors it ends at (its elimination step pos), so lu holds the
+// combined unit-L / U factors of P A as dgetrf_ returns them; perm[k] = the
+// original row chosen as pivot at step k; info as dgetrf_.
+template <int N, int G, int R, class T, bool CPLX>
+__global__ __launch_bounds__(64) void k_factor_test(int num, const double* __restrict__ a_re,
+                                                    const double* __restrict__ a_im, double* __restrict__ lu_re,
+                                                    double* __restrict__ lu_im, int* __restrict__ perm_out,
+                                                    int* __restrict__ info_out) {
+  using RW = Rows<N, G, R>;
+  const int ln = Grp<G>::lane();
+  const int c = blockIdx.x * (64 / G) + (int)threadIdx.x / G;
+  if (c >= num) return;  // whole groups leave together
+  const size_t base = (size_t)c * N * N;
+  T ar[R][N], ai[R][N];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const bool v = RW::valid(ln, q);
+    const int r = RW::row(ln, q);
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      ar[q][j] = v ? (T)a_re[base + (size_t)r * N + j] : T(0);
+      ai[q][j] = (CPLX && v) ? (T)a_im[base + (size_t)r * N + j] : T(0);
+    }
+  }
+  int pos[R];
+  Perm<N> perm;
+#pragma unroll
+  for (int i = 0; i < (N + 3) / 4; ++i) perm.w[i] = 0;
+  T rdr[R], rdi[R];
+  int info = 0;
+  if constexpr (CPLX) lu_cplx<N, G, R, T>(ar, ai, pos, perm, rdr, rdi, info);
+  else lu_real<N, G, R, T>(ar, pos, perm, rdr, info);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (RW::valid(ln, q) && pos[q] >= 0 && pos[q] < N) {
+      const size_t o = base + (size_t)pos[q] * N;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        lu_re[o + j] = (double)ar[q][j];
+        if (CPLX) lu_im[o + j] = (double)ai[q][j];
+      }
+    }
+  }
+  if (ln == 0) {
+    for (int k = 0; k < N; ++k) perm_out[(size_t)c * N + k] = perm.get(k);
+    info_out[c] = info;
+  }
+}
+
+template <int N, int G, int R, class T, bool CPLX>
+static hipError_t launch_one(int num, const double* are, const double* aim, double* lre, double* lim, int* perm,
+                             int* info, hipStream_t s) {
+  constexpr int per_block = 64 / G;
+  const int blocks = (num + per_block - 1) / per_block;
+  hipLaunchKernelGGL((k_factor_test<N, G, R, T, CPLX>), dim3(blocks), dim3(64), 0, s, num, are, aim, lre, lim, perm,
+                     info);
+  return hipGetLastError();
+}
+
+template <int N, int G, int R>
+static hipError_t launch_order(bool fp32, bool cplx, int num, const double* are, const double* aim, double* lre,
+                               double* lim, int* perm, int* info, hipStream_t s) {
+  if (fp32) {
+    return cplx ? launch_one<N, G, R, float, true>(num, are, aim, lre, lim, perm, info, s)
+                : launch_one<N, G, R, float, false>(num, are, aim, lre, lim, perm, info, s);
+  }
+  return cplx ? launch_one<N, G, R, double, true>(num, are, aim, lre, lim, perm, info, s)
+              : launch_one<N, G, R, double, false>(num, are, aim, lre, lim, perm, info, s);
+}
+
+// Device buffers of num matrices of order n (2, 14 or 54); returns
+// hipErrorInvalidValue for any other order.
+hipError_t launch_factor_test(int n, bool fp32, bool cplx, int num, const double* are, const double* aim,
+                              double* lre, double* lim, int* perm, int* info, hipStream_t s) {
+  switch (n) {
+    case 2: return launch_order<2, 1, 2>(fp32, cplx, num, are, aim, lre, lim, perm, info, s);
+    case 14: return launch_order<14, 16, 1>(fp32, cplx, num, are, aim, lre, lim, perm, info, s);
+    case 54: return launch_order<54, 64, 1>(fp32, cplx, num, are, aim, lre, lim, perm, info, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace acc

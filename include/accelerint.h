@@ -207,6 +207,21 @@ int accelerInt_integrate_device_hetero(int NUM14, double *y14_dev, const double 
 int accelerInt_eval_callbacks(int NUM, double t, const double *y_host, const double *var_host,
                               double *dy_host, double *jac_host);
 
+/* Test hook, no counterpart in the reference's API: factors NUM matrices of
+ * order n (2, 14 or 54) on the device with the integrator's row LU for that
+ * order (implicit interchanges, LAPACK pivot rule) and returns them in the
+ * form of LAPACK dgetrf_ / zgetrf_, the calls of RK_Decomp (radau2a.c:273,277):
+ * a_re / a_im are NUM row-major n x n matrices (a_im NULL: real LU; else the
+ * imaginary parts of a complex LU); lu_re / lu_im receive the combined unit-L
+ * / U factors of P A, row-major; ipiv[c*n + k] the 1-based row interchanged
+ * with row k at step k; info[c] the first exactly zero pivot (1-based) or 0.
+ * precision ACCELERINT_PRECISION_MIXED factors the inputs rounded to FP32 in
+ * FP32, as the mixed-precision integrator does.  (At order 14 in FP64 the
+ * integrator itself uses its column LU, which this hook does not run.)
+ * Returns 0 or a negative error. */
+int accelerInt_factor_test(int n, int precision, int NUM, const double *a_re, const double *a_im,
+                           double *lu_re, double *lu_im, int *ipiv, int *info);
+
 /* Mixed precision: number of conditions of the last integration (its last
  * batch of at most NUM-at-initialize conditions) whose FP32-factor run ended
  * with a non-zero code or a non-finite state and were therefore integrated
@@ -284,7 +299,9 @@ int accelerInt_graph_release(void);
 /* Blocks until the library stream is idle; returns 0 or a negative error. */
 int accelerInt_synchronize(void);
 
-/* Last error message recorded by the library ("" if none). */
+/* Last error message recorded by the library ("" if none).  Cleared on entry
+ * to accelerInt_initialize, accelerInt_integrate (and intDriver) and
+ * accelerInt_cleanup, whose void signatures report failure only here. */
 const char *accelerInt_last_error(void);
 
 #ifdef __cplusplus

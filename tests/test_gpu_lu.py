@@ -1,0 +1,172 @@
+"""The integrator's device LU (accelerInt_factor_test) against LAPACK semantics.
+
+The reference factors E1 and E2 with dgetrf_ / zgetrf_ (radau2a.c:273,277);
+the device LU keeps every row in its lane and records the LAPACK position each
+row would hold instead of swapping.  Bar: interchanges (ipiv) and zero-pivot
+reports identical to the oracle's dgetrf / zgetrf index for index, and to the
+value the pivot rule fixes on constructed ties (first maximum in LAPACK's
+current row order; |Re| + |Im| for complex, izamax); FP64 factors within
+1e-10 of max |A| (only the FMA contraction of the update differs); FP32
+factors of FP32-rounded inputs reconstruct P A to 1e-4 of max |A|.
+"""
+import numpy as np
+import pytest
+
+import accelerint_amd as acc
+
+pytestmark = pytest.mark.gpu
+
+ORDERS = [2, 14, 54]
+
+
+def oracle_factor(oracle_lib, a):
+    f = oracle_lib.zgetrf if np.iscomplexobj(a) else oracle_lib.dgetrf
+    out = [f(m) for m in a]
+    return (np.array([o[0] for o in out]), np.array([o[1] for o in out]),
+            np.array([o[2] for o in out]))
+
+
+def random_batch(rng, num, n, cplx):
+    a = rng.standard_normal((num, n, n))
+    if cplx:
+        a = a + 1j * rng.standard_normal((num, n, n))
+    return a
+
+
+def reconstruct(lu, ipiv, a):
+    """max |P a - L U| for LAPACK-form factors."""
+    n = a.shape[0]
+    pa = a.copy()
+    for k in range(n):
+        p = ipiv[k] - 1
+        pa[[k, p]] = pa[[p, k]]
+    L = np.tril(lu, -1) + np.eye(n)
+    U = np.triu(lu)
+    return np.abs(pa - L @ U).max()
+
+
+@pytest.fixture
+def session():
+    acc.lib()
+    yield acc
+    acc.cleanup()
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", ORDERS)
+def test_fp64_factors_match_lapack(session, oracle_lib, n, cplx):
+    rng = np.random.default_rng(1000 + n + 7 * cplx)
+    a = random_batch(rng, 64, n, cplx)
+    lu, ipiv, info = acc.factor_test(a, acc.PRECISION_FP64)
+    lo, io, fo = oracle_factor(oracle_lib, a)
+    assert (info == 0).all() and (fo == 0).all()
+    assert np.array_equal(ipiv, io)
+    scale = np.abs(a).max(axis=(1, 2))
+    err = np.abs(lu - lo).max(axis=(1, 2)) / scale
+    print("n=%d complex=%s: max |LU_gpu - LU_oracle| / max|A| = %.3g" % (n, cplx, err.max()))
+    assert err.max() <= 1e-10
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", ORDERS)
+def test_fp32_factors_reconstruct(session, oracle_lib, n, cplx):
+    rng = np.random.default_rng(2000 + n + 7 * cplx)
+    a = random_batch(rng, 64, n, cplx)
+    a = a.real.astype(np.float32).astype(np.float64) + (1j * a.imag.astype(np.float32).astype(np.float64)
+                                                        if cplx else 0.0)
+    lu, ipiv, info = acc.factor_test(a, acc.PRECISION_MIXED)
+    assert (info == 0).all()
+    res = np.array([reconstruct(lu[c], ipiv[c], a[c]) / np.abs(a[c]).max() for c in range(a.shape[0])])
+    _, io, _ = oracle_factor(oracle_lib, a)
+    same = np.mean([np.array_equal(ipiv[c], io[c]) for c in range(a.shape[0])])
+    print("n=%d complex=%s: max |P A - L U| / max|A| = %.3g; ipiv equal to FP64 LAPACK on %.0f %% of matrices"
+          % (n, cplx, res.max(), 100 * same))
+    assert res.max() <= 1e-4
+    assert same >= 0.9
+
+
+def tie_batch(n, cplx):
+    """Matrices whose pivot choices are fixed by the tie rules; returns
+    (a, expected ipiv prefix)."""
+    rng = np.random.default_rng(3000 + n)
+    mats, expect = [], []
+    # step 0: equal magnitudes in rows i < j of column 0 (-3 and +3): the first wins
+    a = 0.1 * rng.standard_normal((n, n))
+    i, j = (0, 1) if n == 2 else (3, 9)
+    a[i, 0], a[j, 0] = -3.0, 3.0
+    mats.append(a.astype(complex) if cplx else a)
+    expect.append([i + 1])
+    if n >= 3:
+        # step 1: rows 0 and 1 tie in column 1 (5, -5) after step 0 moved row 0
+        # to LAPACK position 2; the first in LAPACK order is row 1 (position 1)
+        a = 0.1 * rng.standard_normal((n, n))
+        a[:, 0] = 0.0
+        a[2, 0] = 1.0
+        a[0, 1], a[1, 1] = 5.0, -5.0
+        mats.append(a.astype(complex) if cplx else a)
+        expect.append([3, 2])
+    if cplx:
+        # |Re| + |Im|: 0.9 + 0.9i (1.8, modulus 1.27) beats 1.5 (modulus 1.5)
+        a = (0.1 * rng.standard_normal((n, n))).astype(complex)
+        i, j = (0, 1) if n == 2 else (3, 7)
+        a[j, 0] = 1.5
+        a[i, 0] = 0.9 + 0.9j
+        mats.append(a)
+        expect.append([i + 1])
+    return np.array(mats), expect
+
+
+@pytest.mark.parametrize("precision", [acc.PRECISION_FP64, acc.PRECISION_MIXED])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", ORDERS)
+def test_pivot_ties_follow_lapack(session, oracle_lib, n, cplx, precision):
+    a, expect = tie_batch(n, cplx)
+    lu, ipiv, info = acc.factor_test(a, precision)
+    _, io, _ = oracle_factor(oracle_lib, a)
+    for c, e in enumerate(expect):
+        assert list(io[c][:len(e)]) == e, "oracle"
+        assert list(ipiv[c][:len(e)]) == e, "device LU, matrix %d: %s" % (c, ipiv[c][:len(e)])
+    if precision == acc.PRECISION_FP64:
+        assert np.array_equal(ipiv, io)
+
+
+@pytest.mark.parametrize("precision", [acc.PRECISION_FP64, acc.PRECISION_MIXED])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", ORDERS)
+def test_zero_pivot_info(session, oracle_lib, n, cplx, precision):
+    rng = np.random.default_rng(4000 + n)
+    mats = []
+    for k in sorted({0, n // 2, n - 1}):
+        a = rng.standard_normal((n, n))
+        a[:, k] = 0.0  # column k zero: the first zero pivot is at step k
+        mats.append(a.astype(complex) if cplx else a)
+    a = np.array(mats)
+    if precision == acc.PRECISION_MIXED:
+        a = a.astype(np.complex64 if cplx else np.float32).astype(np.complex128 if cplx else np.float64)
+    lu, ipiv, info = acc.factor_test(a, precision)
+    _, io, fo = oracle_factor(oracle_lib, a)
+    ks = sorted({0, n // 2, n - 1})
+    assert list(info[:len(ks)]) == [k + 1 for k in ks]
+    assert list(fo[:len(ks)]) == [k + 1 for k in ks]
+    if precision == acc.PRECISION_FP64:
+        assert np.array_equal(ipiv, io)
+
+
+def test_bad_order_is_rejected(session):
+    with pytest.raises(acc.AccelerIntError):
+        acc.factor_test(np.zeros((1, 3, 3)))
+
+
+def test_void_entry_points_clear_the_last_error(session):
+    """accelerInt_initialize / _cleanup report failure only through the last
+    error, so a message left by an earlier failed call must not be read as
+    theirs."""
+    with pytest.raises(acc.AccelerIntError):
+        acc.factor_test(np.zeros((1, 3, 3)))
+    acc.cleanup()
+    assert acc.lib().accelerInt_last_error().decode() == ""
+    with pytest.raises(acc.AccelerIntError):
+        acc.select_mechanism(99)
+    acc.select_mechanism(acc.MECH_VAN_DER_POL)
+    acc.initialize(64, 0)  # raises if the stale message were still reported
+    assert acc.lib().accelerInt_last_error().decode() == ""
