@@ -97,7 +97,7 @@ def lib():
         "accelerInt_synchronize": ([], i),
         "accelerInt_last_error": ([], ctypes.c_char_p),
         "accelerInt_eval_callbacks": ([i, d, v, v, v, v], i),
-        "accelerInt_factor_test": ([i, i, i, v, v, v, v, v, v], i),
+        "accelerInt_factor_test": ([i, i, i, v, v, v, v, v, v, v, v, v, v], i),
         "accelerInt_set_precision": ([i], i),
         "accelerInt_escalated": ([], i),
         "accelerInt_set_schedule": ([i], i),
@@ -493,11 +493,12 @@ def eval_callbacks(NUM: int, t: float, y_host: np.ndarray, var_host: np.ndarray)
     return dy, jac.reshape(NUM, n, n).transpose(0, 2, 1).copy()
 
 
-def factor_test(a: np.ndarray, precision: int = PRECISION_FP64):
+def factor_test(a: np.ndarray, precision: int = PRECISION_FP64, b: np.ndarray = None):
     """The integrator's device LU on matrices a (NUM, n, n), real or complex,
     n in (2, 14, 54); returns (lu, ipiv, info) as LAPACK dgetrf_/zgetrf_ give
     them per matrix: the combined unit-L / U factors of P a (NUM, n, n), the
-    1-based interchanges (NUM, n) and the first zero pivot (NUM,)."""
+    1-based interchanges (NUM, n) and the first zero pivot (NUM,).  With b
+    (NUM, n), also the device solves' x = a^-1 b: (lu, ipiv, info, x)."""
     a = np.asarray(a)
     num, n, _ = a.shape
     cplx = np.iscomplexobj(a)
@@ -507,11 +508,25 @@ def factor_test(a: np.ndarray, precision: int = PRECISION_FP64):
     lim = np.zeros_like(are) if cplx else None
     ipiv = np.zeros((num, n), dtype=np.int32)
     info = np.zeros(num, dtype=np.int32)
-    _check(lib().accelerInt_factor_test(int(n), int(precision), int(num), are.ctypes.data,
-                                        aim.ctypes.data if cplx else None, lre.ctypes.data,
-                                        lim.ctypes.data if cplx else None, ipiv.ctypes.data, info.ctypes.data),
+    bre = bim = xre = xim = None
+    if b is not None:
+        b = np.asarray(b)
+        assert b.shape == (num, n)
+        bre = np.ascontiguousarray(b.real, dtype=np.float64)
+        bim = np.ascontiguousarray(b.imag, dtype=np.float64) if cplx else None
+        xre = np.zeros((num, n))
+        xim = np.zeros((num, n)) if cplx else None
+
+    def ptr(x):
+        return x.ctypes.data if x is not None else None
+
+    _check(lib().accelerInt_factor_test(int(n), int(precision), int(num), ptr(are), ptr(aim), ptr(bre), ptr(bim),
+                                        ptr(lre), ptr(lim), ptr(ipiv), ptr(info), ptr(xre), ptr(xim)),
            "accelerInt_factor_test")
-    return (lre + 1j * lim if cplx else lre), ipiv, info
+    lu = lre + 1j * lim if cplx else lre
+    if b is None:
+        return lu, ipiv, info
+    return lu, ipiv, info, (xre + 1j * xim if cplx else xre)
 
 
 def synchronize():

@@ -393,7 +393,8 @@ hipError_t launch_radau(int num, double t_start, double t_end, double step, cons
 
 // factor_test.hip: the row LU on caller matrices (orders 2, 14, 54).
 hipError_t launch_factor_test(int n, bool fp32, bool cplx, int num, const double* are, const double* aim,
-                              double* lre, double* lim, int* perm, int* info, hipStream_t s);
+                              const double* bre, const double* bim, double* lre, double* lim, int* perm, int* info,
+                              double* xre, double* xim, hipStream_t s);
 
 }  // namespace acc
 
@@ -1444,33 +1445,45 @@ int accelerInt_eval_callbacks(int NUM, double t, const double* y_host, const dou
   return rc;
 }
 
-int accelerInt_factor_test(int n, int precision, int NUM, const double* a_re, const double* a_im, double* lu_re,
-                           double* lu_im, int* ipiv, int* info) {
+int accelerInt_factor_test(int n, int precision, int NUM, const double* a_re, const double* a_im, const double* b_re,
+                           const double* b_im, double* lu_re, double* lu_im, int* ipiv, int* info, double* x_re,
+                           double* x_im) {
   if (!ensure_init()) return -1;
   if ((n != 2 && n != 14 && n != 54) || NUM <= 0 || NUM > (1 << 20) || !a_re || !lu_re || !ipiv || !info ||
-      (a_im && !lu_im) || (precision != ACCELERINT_PRECISION_FP64 && precision != ACCELERINT_PRECISION_MIXED)) {
+      (a_im && !lu_im) || (b_re && !x_re) || (a_im && b_re && (!b_im || !x_im)) ||
+      (precision != ACCELERINT_PRECISION_FP64 && precision != ACCELERINT_PRECISION_MIXED)) {
     set_err("accelerInt_factor_test: bad arguments (n must be 2, 14 or 54; NUM 1..%d)", 1 << 20);
     return -2;
   }
-  const bool cplx = a_im != nullptr;
-  const size_t mat = (size_t)NUM * n * n;
+  const bool cplx = a_im != nullptr, solve = b_re != nullptr;
+  const size_t mat = (size_t)NUM * n * n, vec = (size_t)NUM * n;
   double *d_are = nullptr, *d_aim = nullptr, *d_lre = nullptr, *d_lim = nullptr;
+  double *d_bre = nullptr, *d_bim = nullptr, *d_xre = nullptr, *d_xim = nullptr;
   int *d_perm = nullptr, *d_info = nullptr;
   int rc = 0;
-  std::vector<int> perm((size_t)NUM * n);
+  std::vector<int> perm(vec);
   if (!HIPOK(hipMalloc(&d_are, mat * sizeof(double))) || !HIPOK(hipMalloc(&d_lre, mat * sizeof(double))) ||
       (cplx && (!HIPOK(hipMalloc(&d_aim, mat * sizeof(double))) || !HIPOK(hipMalloc(&d_lim, mat * sizeof(double))))) ||
-      !HIPOK(hipMalloc(&d_perm, (size_t)NUM * n * sizeof(int))) || !HIPOK(hipMalloc(&d_info, NUM * sizeof(int)))) {
+      (solve && (!HIPOK(hipMalloc(&d_bre, vec * sizeof(double))) || !HIPOK(hipMalloc(&d_xre, vec * sizeof(double))))) ||
+      (solve && cplx &&
+       (!HIPOK(hipMalloc(&d_bim, vec * sizeof(double))) || !HIPOK(hipMalloc(&d_xim, vec * sizeof(double))))) ||
+      !HIPOK(hipMalloc(&d_perm, vec * sizeof(int))) || !HIPOK(hipMalloc(&d_info, NUM * sizeof(int)))) {
     rc = -1;
   } else if (!HIPOK(hipMemcpyAsync(d_are, a_re, mat * sizeof(double), hipMemcpyHostToDevice, S.stream)) ||
-             (cplx && !HIPOK(hipMemcpyAsync(d_aim, a_im, mat * sizeof(double), hipMemcpyHostToDevice, S.stream)))) {
+             (cplx && !HIPOK(hipMemcpyAsync(d_aim, a_im, mat * sizeof(double), hipMemcpyHostToDevice, S.stream))) ||
+             (solve && !HIPOK(hipMemcpyAsync(d_bre, b_re, vec * sizeof(double), hipMemcpyHostToDevice, S.stream))) ||
+             (solve && cplx &&
+              !HIPOK(hipMemcpyAsync(d_bim, b_im, vec * sizeof(double), hipMemcpyHostToDevice, S.stream)))) {
     rc = -1;
-  } else if (!hip_ok(launch_factor_test(n, precision == ACCELERINT_PRECISION_MIXED, cplx, NUM, d_are, d_aim, d_lre,
-                                        d_lim, d_perm, d_info, S.stream),
+  } else if (!hip_ok(launch_factor_test(n, precision == ACCELERINT_PRECISION_MIXED, cplx, NUM, d_are, d_aim, d_bre,
+                                        d_bim, d_lre, d_lim, d_perm, d_info, d_xre, d_xim, S.stream),
                      "k_factor_test")) {
     rc = -1;
   } else if (!HIPOK(hipMemcpyAsync(lu_re, d_lre, mat * sizeof(double), hipMemcpyDeviceToHost, S.stream)) ||
              (cplx && !HIPOK(hipMemcpyAsync(lu_im, d_lim, mat * sizeof(double), hipMemcpyDeviceToHost, S.stream))) ||
+             (solve && !HIPOK(hipMemcpyAsync(x_re, d_xre, vec * sizeof(double), hipMemcpyDeviceToHost, S.stream))) ||
+             (solve && cplx &&
+              !HIPOK(hipMemcpyAsync(x_im, d_xim, vec * sizeof(double), hipMemcpyDeviceToHost, S.stream))) ||
              !HIPOK(hipMemcpyAsync(perm.data(), d_perm, perm.size() * sizeof(int), hipMemcpyDeviceToHost, S.stream)) ||
              !HIPOK(hipMemcpyAsync(info, d_info, NUM * sizeof(int), hipMemcpyDeviceToHost, S.stream)) ||
              !HIPOK(hipStreamSynchronize(S.stream))) {
@@ -1502,6 +1515,10 @@ int accelerInt_factor_test(int n, int precision, int NUM, const double* a_re, co
   hipFree(d_aim);
   hipFree(d_lre);
   hipFree(d_lim);
+  hipFree(d_bre);
+  hipFree(d_bim);
+  hipFree(d_xre);
+  hipFree(d_xim);
   hipFree(d_perm);
   hipFree(d_info);
   return rc;

@@ -3,7 +3,10 @@
 // choice, factors and zero-pivot report can be checked one to one against
 // LAPACK dgetrf_/zgetrf_, the routines the reference calls in RK_Decomp
 // (radau2a.c:273,277).  One launcher per order the integrator compiles
-// (2: one system per lane; 14: 16-lane groups; 54: one wave per system).
+// (2: one system per lane; 14: 16-lane groups; 54: one wave per system),
+// and optionally the integrator's triangular solves (solve_real, solve_cplx)
+// on one right-hand side per matrix, the counterpart of dgetrs_ / zgetrs_
+// (radau2a.c:404,417).
 #include <hip/hip_runtime.h>
 
 #include "radau_core.hpp"
@@ -14,12 +17,15 @@ namespace acc {
 // for the real LU).  Row r of the factored matrix is written to the row of
 // the LAPACK factors it ends at (its elimination step pos), so lu holds the
 // combined unit-L / U factors of P A as dgetrf_ returns them; perm[k] = the
-// original row chosen as pivot at step k; info as dgetrf_.
+// original row chosen as pivot at step k; info as dgetrf_.  With b_re set,
+// x = A^-1 b from the factors (b_im / x_im: imaginary parts, complex LU).
 template <int N, int G, int R, class T, bool CPLX>
 __global__ __launch_bounds__(64) void k_factor_test(int num, const double* __restrict__ a_re,
-                                                    const double* __restrict__ a_im, double* __restrict__ lu_re,
+                                                    const double* __restrict__ a_im, const double* __restrict__ b_re,
+                                                    const double* __restrict__ b_im, double* __restrict__ lu_re,
                                                     double* __restrict__ lu_im, int* __restrict__ perm_out,
-                                                    int* __restrict__ info_out) {
+                                                    int* __restrict__ info_out, double* __restrict__ x_re,
+                                                    double* __restrict__ x_im) {
   using RW = Rows<N, G, R>;
   const int ln = Grp<G>::lane();
   const int c = blockIdx.x * (64 / G) + (int)threadIdx.x / G;
@@ -59,37 +65,60 @@ __global__ __launch_bounds__(64) void k_factor_test(int num, const double* __res
     for (int k = 0; k < N; ++k) perm_out[(size_t)c * N + k] = perm.get(k);
     info_out[c] = info;
   }
+  if (b_re) {  // uniform over the launch
+    T br[R], bi[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const bool v = RW::valid(ln, q);
+      const size_t o = (size_t)c * N + RW::row(ln, q);
+      br[q] = v ? (T)b_re[o] : T(0);
+      bi[q] = (CPLX && v) ? (T)b_im[o] : T(0);
+    }
+    if constexpr (CPLX) solve_cplx<N, G, R, T>(ar, ai, pos, perm, rdr, rdi, br, bi);
+    else solve_real<N, G, R, T>(ar, pos, perm, rdr, br);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      if (RW::valid(ln, q)) {
+        const size_t o = (size_t)c * N + RW::row(ln, q);
+        x_re[o] = (double)br[q];
+        if (CPLX) x_im[o] = (double)bi[q];
+      }
+    }
+  }
 }
 
 template <int N, int G, int R, class T, bool CPLX>
-static hipError_t launch_one(int num, const double* are, const double* aim, double* lre, double* lim, int* perm,
-                             int* info, hipStream_t s) {
+static hipError_t launch_one(int num, const double* are, const double* aim, const double* bre, const double* bim,
+                             double* lre, double* lim, int* perm, int* info, double* xre, double* xim,
+                             hipStream_t s) {
   constexpr int per_block = 64 / G;
   const int blocks = (num + per_block - 1) / per_block;
-  hipLaunchKernelGGL((k_factor_test<N, G, R, T, CPLX>), dim3(blocks), dim3(64), 0, s, num, are, aim, lre, lim, perm,
-                     info);
+  hipLaunchKernelGGL((k_factor_test<N, G, R, T, CPLX>), dim3(blocks), dim3(64), 0, s, num, are, aim, bre, bim, lre,
+                     lim, perm, info, xre, xim);
   return hipGetLastError();
 }
 
 template <int N, int G, int R>
-static hipError_t launch_order(bool fp32, bool cplx, int num, const double* are, const double* aim, double* lre,
-                               double* lim, int* perm, int* info, hipStream_t s) {
+static hipError_t launch_order(bool fp32, bool cplx, int num, const double* are, const double* aim,
+                               const double* bre, const double* bim, double* lre, double* lim, int* perm, int* info,
+                               double* xre, double* xim, hipStream_t s) {
   if (fp32) {
-    return cplx ? launch_one<N, G, R, float, true>(num, are, aim, lre, lim, perm, info, s)
-                : launch_one<N, G, R, float, false>(num, are, aim, lre, lim, perm, info, s);
+    return cplx ? launch_one<N, G, R, float, true>(num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s)
+                : launch_one<N, G, R, float, false>(num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
   }
-  return cplx ? launch_one<N, G, R, double, true>(num, are, aim, lre, lim, perm, info, s)
-              : launch_one<N, G, R, double, false>(num, are, aim, lre, lim, perm, info, s);
+  return cplx ? launch_one<N, G, R, double, true>(num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s)
+              : launch_one<N, G, R, double, false>(num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
 }
 
-// Device buffers of num matrices of order n (2, 14 or 54); returns
-// hipErrorInvalidValue for any other order.
+// Device buffers of num matrices of order n (2, 14 or 54); bre = nullptr: no
+// solve.  Returns hipErrorInvalidValue for any other order.
 hipError_t launch_factor_test(int n, bool fp32, bool cplx, int num, const double* are, const double* aim,
-                              double* lre, double* lim, int* perm, int* info, hipStream_t s) {
+                              const double* bre, const double* bim, double* lre, double* lim, int* perm, int* info,
+                              double* xre, double* xim, hipStream_t s) {
   switch (n) {
-    case 2: return launch_order<2, 1, 2>(fp32, cplx, num, are, aim, lre, lim, perm, info, s);
-    case 14: return launch_order<14, 16, 1>(fp32, cplx, num, are, aim, lre, lim, perm, info, s);
-    case 54: return launch_order<54, 64, 1>(fp32, cplx, num, are, aim, lre, lim, perm, info, s);
+    case 2: return launch_order<2, 1, 2>(fp32, cplx, num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
+    case 14: return launch_order<14, 16, 1>(fp32, cplx, num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
+    case 54: return launch_order<54, 64, 1>(fp32, cplx, num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
     default: return hipErrorInvalidValue;
   }
 }

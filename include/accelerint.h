@@ -215,12 +215,16 @@ int accelerInt_eval_callbacks(int NUM, double t, const double *y_host, const dou
  * imaginary parts of a complex LU); lu_re / lu_im receive the combined unit-L
  * / U factors of P A, row-major; ipiv[c*n + k] the 1-based row interchanged
  * with row k at step k; info[c] the first exactly zero pivot (1-based) or 0.
- * precision ACCELERINT_PRECISION_MIXED factors the inputs rounded to FP32 in
- * FP32, as the mixed-precision integrator does.  (At order 14 in FP64 the
- * integrator itself uses its column LU, which this hook does not run.)
- * Returns 0 or a negative error. */
+ * With b_re (and b_im for a complex LU) set, the integrator's triangular
+ * solves, the counterpart of dgetrs_ / zgetrs_ (radau2a.c:404,417), also run
+ * on one right-hand side per matrix (b[c*n + i]) and x_re / x_im receive
+ * A^-1 b; b_re NULL: no solve.  precision ACCELERINT_PRECISION_MIXED factors
+ * and solves with the inputs rounded to FP32 in FP32, as the mixed-precision
+ * integrator does.  (At order 14 in FP64 the integrator itself uses its
+ * column LU, which this hook does not run.)  Returns 0 or a negative error. */
 int accelerInt_factor_test(int n, int precision, int NUM, const double *a_re, const double *a_im,
-                           double *lu_re, double *lu_im, int *ipiv, int *info);
+                           const double *b_re, const double *b_im, double *lu_re, double *lu_im, int *ipiv,
+                           int *info, double *x_re, double *x_im);
 
 /* Mixed precision: number of conditions of the last integration (its last
  * batch of at most NUM-at-initialize conditions) whose FP32-factor run ended

@@ -170,3 +170,33 @@ def test_void_entry_points_clear_the_last_error(session):
     acc.select_mechanism(acc.MECH_VAN_DER_POL)
     acc.initialize(64, 0)  # raises if the stale message were still reported
     assert acc.lib().accelerInt_last_error().decode() == ""
+
+
+@pytest.mark.parametrize("precision", [acc.PRECISION_FP64, acc.PRECISION_MIXED])
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("n", ORDERS)
+def test_solves_match_lapack(session, oracle_lib, n, cplx, precision):
+    """The device triangular solves (dgetrs / zgetrs order) against the
+    oracle's dgetrs / zgetrs on the oracle's factors: FP64 to 1e-10 relative
+    to max |x| (rounding of the reciprocal-pivot products only); FP32
+    residual |A x - b| to 1e-4 of |A| |x|."""
+    rng = np.random.default_rng(5000 + n + 7 * cplx)
+    a = random_batch(rng, 64, n, cplx)
+    b = rng.standard_normal((64, n)) + (1j * rng.standard_normal((64, n)) if cplx else 0.0)
+    if precision == acc.PRECISION_MIXED:
+        a = a.astype(np.complex64 if cplx else np.float32).astype(np.complex128 if cplx else np.float64)
+        b = b.astype(np.complex64 if cplx else np.float32).astype(np.complex128 if cplx else np.float64)
+    lu, ipiv, info, x = acc.factor_test(a, precision, b)
+    assert (info == 0).all()
+    solve = oracle_lib.zgetrs if cplx else oracle_lib.dgetrs
+    lo, io, _ = oracle_factor(oracle_lib, a)
+    xo = np.array([solve(lo[c], io[c], b[c]) for c in range(a.shape[0])])
+    if precision == acc.PRECISION_FP64:
+        err = (np.abs(x - xo).max(axis=1) / np.abs(xo).max(axis=1)).max()
+        print("n=%d complex=%s FP64: max |x_gpu - x_oracle| / max|x| = %.3g" % (n, cplx, err))
+        assert err <= 1e-10
+    else:
+        res = np.abs(np.einsum("cij,cj->ci", a, x) - b).max(axis=1)
+        scale = np.einsum("cij,cj->ci", np.abs(a), np.abs(x)).max(axis=1)
+        print("n=%d complex=%s FP32: max |A x - b| / (|A| |x|) = %.3g" % (n, cplx, (res / scale).max()))
+        assert (res / scale).max() <= 1e-4
