@@ -1,12 +1,13 @@
-// Device LU without integration: the integrator's row-LU routines (lu_real,
-// lu_cplx of radau_core.hpp) run on caller matrices so that their pivot
+// Device LU without integration: the LU routines the integrator runs at each
+// order and precision (radau_core.hpp: the row LU lu_real / lu_cplx with
+// implicit interchanges; at order 14 in FP64 the column LU clu_real /
+// clu_cplx with explicit ones) on caller matrices, so that their pivot
 // choice, factors and zero-pivot report can be checked one to one against
 // LAPACK dgetrf_/zgetrf_, the routines the reference calls in RK_Decomp
-// (radau2a.c:273,277).  One launcher per order the integrator compiles
-// (2: one system per lane; 14: 16-lane groups; 54: one wave per system),
-// and optionally the integrator's triangular solves (solve_real, solve_cplx)
-// on one right-hand side per matrix, the counterpart of dgetrs_ / zgetrs_
-// (radau2a.c:404,417).
+// (radau2a.c:273,277); optionally also the matching triangular solves on one
+// right-hand side per matrix, the counterpart of dgetrs_ / zgetrs_
+// (radau2a.c:404,417).  Orders 2 (one system per lane), 14 (16-lane
+// groups) and 54 (one wave per system).
 #include <hip/hip_runtime.h>
 
 #include "radau_core.hpp"
@@ -87,6 +88,63 @@ __global__ __launch_bounds__(64) void k_factor_test(int num, const double* __res
   }
 }
 
+// Order 14 in FP64: the column LU (lane j holds column j, rows physically
+// interchanged; rmap byte k = the original row now at position k, which is
+// also the row chosen at step k), then the factors turned into rows through
+// LDS for the row-layout solves, as the integrator does.
+template <bool CPLX>
+__global__ __launch_bounds__(64) void k_factor_test_col(int num, const double* __restrict__ a_re,
+                                                        const double* __restrict__ a_im,
+                                                        const double* __restrict__ b_re,
+                                                        const double* __restrict__ b_im, double* __restrict__ lu_re,
+                                                        double* __restrict__ lu_im, int* __restrict__ perm_out,
+                                                        int* __restrict__ info_out, double* __restrict__ x_re,
+                                                        double* __restrict__ x_im) {
+  constexpr int N = 14, G = 16;
+  __shared__ double scratch[64 / G][N * N];
+  const int ln = Grp<G>::lane();
+  const int g = (int)threadIdx.x / G;
+  const int c = blockIdx.x * (64 / G) + g;
+  if (c >= num) return;  // whole groups leave together
+  const size_t base = (size_t)c * N * N;
+  double ar[N], ai[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    ar[i] = ln < N ? a_re[base + (size_t)i * N + ln] : 0.0;
+    ai[i] = (CPLX && ln < N) ? a_im[base + (size_t)i * N + ln] : 0.0;
+  }
+  RowMap<N> rmap;
+  double rdr = 0.0, rdi = 0.0;
+  int info = 0;
+  if constexpr (CPLX) clu_cplx<N, G>(ar, ai, rmap, rdr, rdi, info);
+  else clu_real<N, G>(ar, rmap, rdr, info);
+  if (ln < N) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      lu_re[base + (size_t)i * N + ln] = ar[i];
+      if (CPLX) lu_im[base + (size_t)i * N + ln] = ai[i];
+    }
+  }
+  if (ln == 0) {
+    for (int k = 0; k < N; ++k) perm_out[(size_t)c * N + k] = rmap.get(k);
+    info_out[c] = info;
+  }
+  if (b_re) {  // uniform over the launch
+    double rr[N], ri[N];
+    transpose_cols_to_rows<N, G>(ar, rr, scratch[g]);
+    if (CPLX) transpose_cols_to_rows<N, G>(ai, ri, scratch[g]);
+    const int src = rmap.get(ln);
+    double br = ln < N ? b_re[(size_t)c * N + ln] : 0.0;
+    double bi = (CPLX && ln < N) ? b_im[(size_t)c * N + ln] : 0.0;
+    if constexpr (CPLX) rsolve_cplx<N, G>(rr, ri, src, rdr, rdi, br, bi);
+    else rsolve_real<N, G>(rr, src, rdr, br);
+    if (ln < N) {
+      x_re[(size_t)c * N + ln] = br;
+      if (CPLX) x_im[(size_t)c * N + ln] = bi;
+    }
+  }
+}
+
 template <int N, int G, int R, class T, bool CPLX>
 static hipError_t launch_one(int num, const double* are, const double* aim, const double* bre, const double* bim,
                              double* lre, double* lim, int* perm, int* info, double* xre, double* xim,
@@ -117,7 +175,19 @@ hipError_t launch_factor_test(int n, bool fp32, bool cplx, int num, const double
                               double* xre, double* xim, hipStream_t s) {
   switch (n) {
     case 2: return launch_order<2, 1, 2>(fp32, cplx, num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
-    case 14: return launch_order<14, 16, 1>(fp32, cplx, num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
+    case 14:
+      if (!fp32) {
+        const int blocks = (num + 3) / 4;
+        if (cplx)
+          hipLaunchKernelGGL(k_factor_test_col<true>, dim3(blocks), dim3(64), 0, s, num, are, aim, bre, bim, lre, lim,
+                             perm, info, xre, xim);
+        else
+          hipLaunchKernelGGL(k_factor_test_col<false>, dim3(blocks), dim3(64), 0, s, num, are, aim, bre, bim, lre,
+                             lim, perm, info, xre, xim);
+        return hipGetLastError();
+      }
+      return cplx ? launch_one<14, 16, 1, float, true>(num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s)
+                  : launch_one<14, 16, 1, float, false>(num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
     case 54: return launch_order<54, 64, 1>(fp32, cplx, num, are, aim, bre, bim, lre, lim, perm, info, xre, xim, s);
     default: return hipErrorInvalidValue;
   }

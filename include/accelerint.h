@@ -208,8 +208,10 @@ int accelerInt_eval_callbacks(int NUM, double t, const double *y_host, const dou
                               double *dy_host, double *jac_host);
 
 /* Test hook, no counterpart in the reference's API: factors NUM matrices of
- * order n (2, 14 or 54) on the device with the integrator's row LU for that
- * order (implicit interchanges, LAPACK pivot rule) and returns them in the
+ * order n (2, 14 or 54) on the device with the LU the integrator runs at that
+ * order and precision (the row LU with implicit interchanges; at order 14 in
+ * FP64 the column LU with explicit ones; LAPACK's pivot rule in both) and
+ * returns them in the
  * form of LAPACK dgetrf_ / zgetrf_, the calls of RK_Decomp (radau2a.c:273,277):
  * a_re / a_im are NUM row-major n x n matrices (a_im NULL: real LU; else the
  * imaginary parts of a complex LU); lu_re / lu_im receive the combined unit-L
@@ -220,8 +222,7 @@ int accelerInt_eval_callbacks(int NUM, double t, const double *y_host, const dou
  * on one right-hand side per matrix (b[c*n + i]) and x_re / x_im receive
  * A^-1 b; b_re NULL: no solve.  precision ACCELERINT_PRECISION_MIXED factors
  * and solves with the inputs rounded to FP32 in FP32, as the mixed-precision
- * integrator does.  (At order 14 in FP64 the integrator itself uses its
- * column LU, which this hook does not run.)  Returns 0 or a negative error. */
+ * integrator does.  Returns 0 or a negative error. */
 int accelerInt_factor_test(int n, int precision, int NUM, const double *a_re, const double *a_im,
                            const double *b_re, const double *b_im, double *lu_re, double *lu_im, int *ipiv,
                            int *info, double *x_re, double *x_im);
