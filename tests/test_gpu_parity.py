@@ -23,6 +23,25 @@ def weighted_err(y, yref, atol=ATOL, rtol=RTOL):
     return np.abs(y - yref) / (atol + rtol * np.abs(yref))
 
 
+LU = acc.CNT_NAMES.index("lu")
+
+
+def assert_same_step_sequence(cnt_g, cnt_o, label, max_lu_frac=0.05):
+    """FP64 against the oracle: every event count but the LU count identical
+    for every condition - the same accepted and rejected steps, Newton
+    iterations and failures, dydt, Jacobian and error-estimate calls.  The LU
+    count may differ by a few on a few conditions: the LU-reuse test
+    (radau2a.c:709, Hnew / H inside [Qmin, Qmax]) flips where Hnew / H lands
+    within rounding of the window's edge, which changes no step decision."""
+    others = [i for i in range(cnt_g.shape[1]) if i != LU]
+    differ = (cnt_g[:, others] != cnt_o[:, others]).any(1)
+    dl = np.abs(cnt_g[:, LU] - cnt_o[:, LU])
+    print("%s: %d conditions with any other event count differing; LU count differs on %.2f %% (max %d)"
+          % (label, differ.sum(), 100 * (dl > 0).mean(), dl.max()))
+    assert not differ.any()
+    assert dl.max() <= 3 and (dl > 0).mean() <= max_lu_frac
+
+
 @pytest.fixture
 def session():
     acc.lib()
@@ -54,9 +73,7 @@ def test_vdp_config0_parity(session, oracle_lib):
     print("vdp weighted err median %.3g max %.3g" % (np.median(we), we.max()))
     assert we.max() <= 10.0
     cnt_g = acc.get_counters(num).astype(np.int64)
-    same = (cnt_g == cnt_o).all(1).mean()
-    print("conditions with identical event counters: %.3f" % same)
-    assert same > 0.5
+    assert_same_step_sequence(cnt_g, cnt_o, "vdp fp64")
 
 
 def test_vdp_int_driver_matches_integrate(session):
@@ -140,7 +157,7 @@ def test_kinetics14_parity(session, oracle_lib, net14):
     print("kinetics14 weighted err median %.3g p99 %.3g max %.3g" % (np.median(we), np.percentile(we, 99), we.max()))
     assert we.max() <= 10.0
     cnt_g = acc.get_counters(num).astype(np.int64)
-    print("conditions with identical event counters: %.3f" % (cnt_g == cnt_o).all(1).mean())
+    assert_same_step_sequence(cnt_g, cnt_o, "kinetics14 fp64")
 
 
 def _kin_parity(oracle_lib, mech, num, steps, dt, seed, T_range, precision, label, min_same=None):
@@ -171,6 +188,8 @@ def _kin_parity(oracle_lib, mech, num, steps, dt, seed, T_range, precision, labe
         assert we.max() <= 10.0
     if min_same is not None:
         assert same >= min_same
+    if precision == acc.PRECISION_FP64:
+        assert_same_step_sequence(cnt_g, cnt_o, label)
     return cnt_g, cnt_o
 
 
