@@ -500,6 +500,8 @@ def factor_test(a: np.ndarray, precision: int = PRECISION_FP64, b: np.ndarray = 
     1-based interchanges (NUM, n) and the first zero pivot (NUM,).  With b
     (NUM, n), also the device solves' x = a^-1 b: (lu, ipiv, info, x)."""
     a = np.asarray(a)
+    if a.ndim != 3 or a.shape[1] != a.shape[2]:
+        raise ValueError("factor_test: a must have shape (NUM, n, n), got %s" % (a.shape,))
     num, n, _ = a.shape
     cplx = np.iscomplexobj(a)
     are = np.ascontiguousarray(a.real, dtype=np.float64)
@@ -511,7 +513,8 @@ def factor_test(a: np.ndarray, precision: int = PRECISION_FP64, b: np.ndarray = 
     bre = bim = xre = xim = None
     if b is not None:
         b = np.asarray(b)
-        assert b.shape == (num, n)
+        if b.shape != (num, n):
+            raise ValueError("factor_test: b must have shape (%d, %d), got %s" % (num, n, b.shape))
         bre = np.ascontiguousarray(b.real, dtype=np.float64)
         bim = np.ascontiguousarray(b.imag, dtype=np.float64) if cplx else None
         xre = np.zeros((num, n))
