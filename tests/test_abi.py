@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 import accelerint_amd as acc
@@ -83,3 +84,12 @@ def test_check_error_passes_clean_codes():
             "print('returned')" % acc.LIB_PATH)
     r = subprocess.run([sys.executable, "-c", prog], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0 and "returned" in r.stdout
+
+
+def test_factor_test_rejects_bad_shapes_before_any_device_call():
+    with pytest.raises(ValueError):
+        acc.factor_test(np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        acc.factor_test(np.zeros((1, 3, 4)))
+    with pytest.raises(ValueError):
+        acc.factor_test(np.zeros((2, 14, 14)), b=np.zeros((2, 13)))
