@@ -454,6 +454,15 @@ def test_fd_jacobian_parity(session, oracle_lib, net14, system, precision):
           % (system, precision, we.max(), same))
     if precision == 0:
         assert we.max() <= 10.0
+        # reported, not held to the analytic-Jacobian event-sequence bar: the
+        # difference quotient magnifies last-bit differences of dydt (FMA
+        # contraction) by about |f| / r ~ 1e8, which moves Newton convergence
+        # decisions at their margins (measured: van der Pol 6 % of conditions,
+        # 14-state none beyond the LU count)
+        others = [i for i in range(cnt.shape[1]) if i != LU]
+        dl = np.abs(cnt[:, LU] - cnt_o[:, LU])
+        print("%s FD fp64: other event counts differ on %.2f %%; LU count on %.2f %% (max %d)"
+              % (system, 100 * (cnt[:, others] != cnt_o[:, others]).any(1).mean(), 100 * (dl > 0).mean(), dl.max()))
     else:
         arbitrate("%s FD mixed" % system, y, yo, oracle_lib, om, y0, pr, dt, steps)
 
