@@ -53,7 +53,7 @@ EXPORTS = (
     "accelerInt_set_schedule", "accelerInt_get_schedule", "read_initial_conditions",
     "accelerInt_read_initial_conditions", "accelerInt_write_log", "accelerInt_load_kinetics",
     "accelerInt_set_launch_shape", "accelerInt_get_launch_shape", "accelerInt_autotune", "accelerInt_load_tuning",
-    "accelerInt_set_jacobian", "accelerInt_factor_test",
+    "accelerInt_set_jacobian", "accelerInt_factor_test", "accelerInt_set_fd_order",
 )
 
 
@@ -106,6 +106,7 @@ def lib():
         "accelerInt_load_kinetics": ([ctypes.c_char_p], i),
         "accelerInt_set_launch_shape": ([i, i, i], i),
         "accelerInt_set_jacobian": ([i], i),
+        "accelerInt_set_fd_order": ([i], i),
         "accelerInt_get_launch_shape": ([i, i], i),
         "accelerInt_autotune": ([i, d, d, d, v, v, ctypes.c_char_p, v], i),
         "accelerInt_load_tuning": ([ctypes.c_char_p], i),
@@ -212,6 +213,11 @@ def set_jacobian(mode: int):
     """JACOBIAN_ANALYTIC (default) or JACOBIAN_FINITE_DIFFERENCE (generic/fd_jacob.c;
     van der Pol and the 14-state network)."""
     _check(lib().accelerInt_set_jacobian(int(mode)), "accelerInt_set_jacobian")
+
+
+def set_fd_order(order: int):
+    """Order of the finite-difference Jacobian (FD_ORD of generic/fd_jacob.c): 1, 2, 4 or 6."""
+    _check(lib().accelerInt_set_fd_order(int(order)), "accelerInt_set_fd_order")
 
 
 def set_launch_shape(n_states: int, precision: int, waves_per_simd: int):

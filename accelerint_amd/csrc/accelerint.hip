@@ -408,6 +408,7 @@ struct State {
   int n = 2;
   int precision = ACCELERINT_PRECISION_FP64;
   int jacobian = ACCELERINT_JACOBIAN_ANALYTIC;
+  int fd_order = 1;  // finite-difference order of the FD Jacobian (FD_ORD)
   int learned = 0;          // learned Newton start on (the network is looked up per launch)
   float* own_log_x = nullptr;  // library-owned training-sample log (accelerInt_start_log)
   float* own_log_z = nullptr;
@@ -620,11 +621,11 @@ hipError_t launch_mt(int num, double t_start, double t_end, double step, const d
     if (S.mech == ACCELERINT_MECH_VAN_DER_POL)
       return launch_radau<MechFD<MechVdP>, 1, 2, 256, ACC_VDP_WPE, MT>(
           num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P,
-          MechFD<MechVdP>::Params{MechVdP::Params{}, S.P.atol, S.P.rtol}, s, cost);
+          MechFD<MechVdP>::Params{MechVdP::Params{}, S.P.atol, S.P.rtol, S.fd_order}, s, cost);
     if (S.mech == ACCELERINT_MECH_KINETICS && S.n == 14)
       return launch_radau<MechFD<MechKinetics14>, 16, 1, 128, ACC_K14_WPE, MT>(
           num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P,
-          MechFD<MechKinetics14>::Params{S.kp14, S.P.atol, S.P.rtol}, s, cost);
+          MechFD<MechKinetics14>::Params{S.kp14, S.P.atol, S.P.rtol, S.fd_order}, s, cost);
     g_launch_msg = "the finite-difference Jacobian is compiled for van der Pol and the 14-state network only";
     return hipErrorInvalidValue;
   }
@@ -1008,6 +1009,15 @@ int accelerInt_set_jacobian(int mode) {
     return -1;
   }
   S.jacobian = mode;
+  return 0;
+}
+
+int accelerInt_set_fd_order(int order) {
+  if (order != 1 && order != 2 && order != 4 && order != 6) {
+    set_err("finite-difference order must be 1, 2, 4 or 6 (got %d)", order);
+    return -1;
+  }
+  S.fd_order = order;
   return 0;
 }
 

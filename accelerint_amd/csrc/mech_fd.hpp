@@ -1,13 +1,16 @@
 // Finite-difference Jacobian over any mechanism's dydt: the reference's
-// generic/fd_jacob.c:24-127 with FD_ORD 1 (its default), for callbacks that
-// come without an analytic Jacobian.  Per column j, with ewt_i = atol +
-// rtol |y_i| (the tolerances themselves, as the reference uses them):
+// generic/fd_jacob.c:24-127 for FD_ORD 1 (its default), 2, 4 and 6, for
+// callbacks that come without an analytic Jacobian.  Per column j, with
+// ewt_i = atol + rtol |y_i| (the tolerances themselves, as the reference uses
+// them):
 //   fac = sqrt(sum_i (ewt_i f_i)^2 / N),  r0 = 1000 rtol eps N fac,
 //   r   = max(sqrt(eps) |y_j|, r0 / ewt_j),
-//   J(:, j) = (f(y + r e_j) - f(y)) / r.
-// One dydt per column plus one at y: N + 1 evaluations per Jacobian.  Each
-// lane writes the entries of the rows it owns, into the same place the base
-// mechanism's analytic Jacobian goes (registers, or the group's LDS rows).
+//   order 1:    J(:, j) = (f(y + r e_j) - f(y)) / r,
+//   order 2m:   J(:, j) = (sum_k w_k f(y + x_k r e_j)) / r over the central
+//               stencil of order 2m (fd_stencil below), summed in k order.
+// Order 1 takes N + 1 dydt evaluations per Jacobian, order p takes p N + 1.
+// Each lane writes the entries of the rows it owns, into the same place the
+// base mechanism's analytic Jacobian goes (registers, or the group's LDS rows).
 #pragma once
 
 #include <float.h>
@@ -16,6 +19,34 @@
 
 namespace acc {
 
+// Central-difference stencils of orders 2, 4 and 6: offsets x_k (in units of
+// the increment r) and weights w_k, the textbook values that make
+// sum_k w_k f(x_k r) / r exact for polynomials of degree <= order (weights
+// antisymmetric, sum_k w_k x_k = 1, higher moments zero up to the order).
+__host__ __device__ inline void fd_stencil(int order, int k, double& x, double& w) {
+  switch (order) {
+    case 2: {
+      x = k == 0 ? -1.0 : 1.0;
+      w = k == 0 ? -0.5 : 0.5;
+      break;
+    }
+    case 4: {
+      const double xs[4] = {-2.0, -1.0, 1.0, 2.0};
+      const double ws[4] = {1.0 / 12.0, -2.0 / 3.0, 2.0 / 3.0, -1.0 / 12.0};
+      x = xs[k];
+      w = ws[k];
+      break;
+    }
+    default: {  // 6
+      const double xs[6] = {-3.0, -2.0, -1.0, 1.0, 2.0, 3.0};
+      const double ws[6] = {-1.0 / 60.0, 3.0 / 20.0, -3.0 / 4.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
+      x = xs[k];
+      w = ws[k];
+      break;
+    }
+  }
+}
+
 template <class Base>
 struct MechFD {
   static constexpr int N = Base::N;
@@ -23,6 +54,7 @@ struct MechFD {
   struct Params {
     typename Base::Params base;
     double atol, rtol;
+    int order;  // 1, 2, 4 or 6
   };
   static size_t lds_bytes(const Params& p, int groups, bool scratch) {
     return Base::lds_bytes(p.base, groups, scratch);
@@ -30,10 +62,11 @@ struct MechFD {
   __device__ __forceinline__ static void block_init(const Params& p, double* lds) { Base::block_init(p.base, lds); }
   struct Ctx : Base::Ctx {
     double atol, rtol;
+    int order;
     __device__ Ctx(const Params& p, int g, double* lds, bool scratch)
-        : Base::Ctx(p.base, g, lds, scratch), atol(p.atol), rtol(p.rtol) {}
+        : Base::Ctx(p.base, g, lds, scratch), atol(p.atol), rtol(p.rtol), order(p.order) {}
     __device__ Ctx(const Params& p, const double* tables, double* group, bool scratch)
-        : Base::Ctx(p.base, tables, group, scratch), atol(p.atol), rtol(p.rtol) {}
+        : Base::Ctx(p.base, tables, group, scratch), atol(p.atol), rtol(p.rtol), order(p.order) {}
   };
 
   template <int G, int R>
@@ -68,11 +101,28 @@ struct MechFD {
       const double ej = c.atol + c.rtol * fabs(yj);
       const double r = fmax(srur * fabs(yj), r0 / ej);
       double yp[R], ft[R], col[R];
+      if (c.order == 1) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) yp[q] = (RW::valid(ln, q) && RW::row(ln, q) == j) ? yj + r : y[q];
-      Base::template dydt<G, R>(c, t, pr, yp, ft);
+        for (int q = 0; q < R; ++q) yp[q] = (RW::valid(ln, q) && RW::row(ln, q) == j) ? yj + r : y[q];
+        Base::template dydt<G, R>(c, t, pr, yp, ft);
 #pragma unroll
-      for (int q = 0; q < R; ++q) col[q] = (ft[q] - f0[q]) / r;
+        for (int q = 0; q < R; ++q) col[q] = (ft[q] - f0[q]) / r;
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) col[q] = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < c.order; ++k) {
+          double xk, wk;
+          fd_stencil(c.order, k, xk, wk);
+#pragma unroll
+          for (int q = 0; q < R; ++q) yp[q] = (RW::valid(ln, q) && RW::row(ln, q) == j) ? yj + xk * r : y[q];
+          Base::template dydt<G, R>(c, t, pr, yp, ft);
+#pragma unroll
+          for (int q = 0; q < R; ++q) col[q] += wk * ft[q];
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) col[q] /= r;
+      }
       put(j, col);
     }
   }

@@ -125,6 +125,11 @@ int accelerInt_set_precision(int mode);
 #define ACCELERINT_JACOBIAN_ANALYTIC 0
 #define ACCELERINT_JACOBIAN_FINITE_DIFFERENCE 1
 int accelerInt_set_jacobian(int mode);
+/* Order of the finite-difference Jacobian, the reference's compile-time
+ * FD_ORD (generic/fd_jacob.c:14,33-66): 1 (default, forward difference), or
+ * 2, 4, 6 (central differences, order x N + 1 dydt calls per Jacobian).
+ * Returns 0 or -1 for any other order. */
+int accelerInt_set_fd_order(int order);
 
 /* Launch shapes (no reference counterpart): the waves per SIMD the integration
  * kernel's register allocation targets.  van der Pol: 1, 2 (default) or 4; the

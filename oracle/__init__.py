@@ -29,6 +29,7 @@ NCNT = 10
 CNT_NAMES = ("step", "accept", "reject", "newton_fail", "lu", "lu_fail", "newton", "f", "jac", "es")
 MECH_VDP, MECH_LINEAR, MECH_KINETICS = 0, 1, 2
 FD_JACOBIAN = 256  # ORA_FD_JACOBIAN: eval_jacob by generic/fd_jacob.c
+FD_ORDER_SHIFT = 9  # ORA_FD_ORDER_SHIFT: FD_ORD in bits 9-11 of the kind
 
 
 def build(force: bool = False) -> str:
@@ -56,6 +57,9 @@ def lib():
         L.ora_batch.restype = None
         L.ora_eval.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                ctypes.c_double, dp, dp]
+        L.ora_fd_stencil.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]
+        L.ora_fd_stencil.restype = None
         L.ora_fd_jacob.argtypes = [ctypes.c_int, vp, ctypes.c_int, ctypes.c_double,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp]
         L.ora_dgetrf.argtypes = [ctypes.c_int, dp, ip, ip]
@@ -126,7 +130,7 @@ class Mechanism:
 
 
 def integrate(mech: Mechanism, t, t_end, pr, y_colmajor, atol=1e-10, rtol=1e-6,
-              const_time_step=False, nthreads=0, fd_jacobian=False):
+              const_time_step=False, nthreads=0, fd_jacobian=False, fd_order=1):
     """One intDriver call (solver_generic.c:28): y is column-major (n, num),
     updated copy returned with codes and counters.  fd_jacobian: the Jacobian
     by generic/fd_jacob.c instead of the analytic callback."""
@@ -136,7 +140,7 @@ def integrate(mech: Mechanism, t, t_end, pr, y_colmajor, atol=1e-10, rtol=1e-6,
     pr = np.ascontiguousarray(pr, dtype=np.float64)
     codes = np.zeros(num, dtype=np.int32)
     cnt = np.zeros((num, NCNT), dtype=np.int64)
-    kind = mech.kind | (FD_JACOBIAN if fd_jacobian else 0)
+    kind = mech.kind | ((FD_JACOBIAN | (int(fd_order) << FD_ORDER_SHIFT)) if fd_jacobian else 0)
     lib().ora_batch(kind, mech.ctx_ptr, n, atol, rtol, int(const_time_step), num, t, t_end,
                     _dp(pr), _dp(y), _ip(codes),
                     cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), int(nthreads))
@@ -172,11 +176,24 @@ def jacobian(mech, t, pr, y):
     return out.reshape(mech.n, mech.n, order="F")
 
 
-def fd_jacobian(mech, t, pr, y, atol=1e-10, rtol=1e-6):
+def fd_jacobian(mech, t, pr, y, atol=1e-10, rtol=1e-6, order=1):
     out = np.zeros(mech.n * mech.n)
     y = np.ascontiguousarray(y, dtype=np.float64)
-    lib().ora_fd_jacob(mech.kind, mech.ctx_ptr, mech.n, atol, rtol, t, pr, _dp(y), _dp(out))
+    lib().ora_fd_jacob(mech.kind | (int(order) << FD_ORDER_SHIFT), mech.ctx_ptr, mech.n, atol, rtol, t, pr,
+                       _dp(y), _dp(out))
     return out.reshape(mech.n, mech.n, order="F")
+
+
+def fd_stencil(order):
+    """(offsets, weights) of the central-difference stencil of FD_ORD order (2, 4, 6)."""
+    xs, ws = [], []
+    for k in range(order):
+        x = ctypes.c_double()
+        w = ctypes.c_double()
+        lib().ora_fd_stencil(int(order), k, ctypes.byref(x), ctypes.byref(w))
+        xs.append(x.value)
+        ws.append(w.value)
+    return np.array(xs), np.array(ws)
 
 
 def kinetics_rates(mech, T, P, Y):

@@ -279,7 +279,9 @@ static void make_problem(ora_problem *p, int mech, const void *ctx, int n, doubl
     p->const_time_step = cts;
     p->ctx = ctx;
     p->fd_jac = (mech & ORA_FD_JACOBIAN) != 0;
-    mech &= ~ORA_FD_JACOBIAN;
+    p->fd_order = (mech >> ORA_FD_ORDER_SHIFT) & 7;
+    if (p->fd_order == 0) p->fd_order = 1;
+    mech &= 0xff;
     switch (mech) {
         case 0: p->dydt = ora_vdp_dydt; p->jac = ora_vdp_jac; break;
         case 1: p->dydt = ora_linear_dydt; p->jac = ora_linear_jac; break;
@@ -304,7 +306,23 @@ void ora_eval(int mech, const void *ctx, int n, int what, double t, double pr, c
     else p.jac(t, pr, y, out, p.ctx);
 }
 
-/* generic/fd_jacob.c:24-127 with FD_ORD == 1 */
+/* Central-difference stencils of generic/fd_jacob.c:33-66 (FD_ORD 2, 4, 6):
+ * offsets in units of the increment and weights, the textbook central
+ * differences (antisymmetric weights, first moment 1, higher moments zero up
+ * to the order). */
+static void fd_stencil(int order, int k, double *x, double *w) {
+    static const double x2[2] = {-1.0, 1.0}, w2[2] = {-0.5, 0.5};
+    static const double x4[4] = {-2.0, -1.0, 1.0, 2.0};
+    static const double w4[4] = {1.0 / 12.0, -2.0 / 3.0, 2.0 / 3.0, -1.0 / 12.0};
+    static const double x6[6] = {-3.0, -2.0, -1.0, 1.0, 2.0, 3.0};
+    static const double w6[6] = {-1.0 / 60.0, 3.0 / 20.0, -3.0 / 4.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
+    if (order == 2) { *x = x2[k]; *w = w2[k]; }
+    else if (order == 4) { *x = x4[k]; *w = w4[k]; }
+    else { *x = x6[k]; *w = w6[k]; }
+}
+
+/* generic/fd_jacob.c:24-127: FD_ORD 1 forward differences (:94-101), 2/4/6
+ * central stencils summed in stencil order and divided by r (:103-119) */
 void ora_problem_fd_jacob(const ora_problem *p, double t, double pr, const double *cy, double *jac) {
     const int n = p->n;
     double y[n], dy[n], ewt[n], ftemp[n];
@@ -319,12 +337,27 @@ void ora_problem_fd_jacob(const ora_problem *p, double t, double pr, const doubl
     for (int j = 0; j < n; ++j) {
         double yj = y[j];
         double r = fmax(srur * fabs(yj), r0 / ewt[j]);
-        y[j] = yj + r;
-        p->dydt(t, pr, y, ftemp, p->ctx);
-        for (int i = 0; i < n; ++i) jac[i + n * j] = (ftemp[i] - dy[i]) / r;
+        if (p->fd_order <= 1) {
+            y[j] = yj + r;
+            p->dydt(t, pr, y, ftemp, p->ctx);
+            for (int i = 0; i < n; ++i) jac[i + n * j] = (ftemp[i] - dy[i]) / r;
+        } else {
+            for (int i = 0; i < n; ++i) jac[i + n * j] = 0.0;
+            for (int k = 0; k < p->fd_order; ++k) {
+                double xk, wk;
+                fd_stencil(p->fd_order, k, &xk, &wk);
+                y[j] = yj + xk * r;
+                p->dydt(t, pr, y, ftemp, p->ctx);
+                for (int i = 0; i < n; ++i) jac[i + n * j] += wk * ftemp[i];
+            }
+            for (int i = 0; i < n; ++i) jac[i + n * j] /= r;
+        }
         y[j] = yj;
     }
 }
+
+/* ctypes: the stencil of an order, for the moment checks of the tests */
+void ora_fd_stencil(int order, int k, double *x, double *w) { fd_stencil(order, k, x, w); }
 
 void ora_fd_jacob(int mech, const void *ctx, int n, double atol, double rtol, double t, double pr,
                   const double *cy, double *jac) {

@@ -40,11 +40,14 @@ typedef struct {
     ora_jac_fn jac;
     const void *ctx;
     int fd_jac;            /* 1: eval_jacob is generic/fd_jacob.c over dydt */
+    int fd_order;          /* FD_ORD of generic/fd_jacob.c:14: 1, 2, 4 or 6 */
 } ora_problem;
 
 /* OR-ed into a batch driver's mechanism kind: use the finite-difference
  * Jacobian (generic/fd_jacob.c) instead of the analytic one. */
 #define ORA_FD_JACOBIAN 256
+/* Finite-difference order (1, 2, 4, 6) in bits 9-11 of the kind; 0 means 1. */
+#define ORA_FD_ORDER_SHIFT 9
 
 /* Per-condition event counters (algorithmic work accounting). */
 enum {
@@ -126,7 +129,8 @@ void ora_batch(int mech, const void *ctx, int n, double atol, double rtol, int c
                int *codes, int64_t *counters, int nthreads);
 void ora_eval(int mech, const void *ctx, int n, int what, double t, double pr, const double *y,
               double *out);
-/* Finite-difference Jacobian, generic/fd_jacob.c:24-127 (FD_ORD 1). */
+/* Finite-difference Jacobian, generic/fd_jacob.c:24-127 (FD_ORD from the
+ * kind's order bits). */
 void ora_fd_jacob(int mech, const void *ctx, int n, double atol, double rtol, double t, double pr,
                   const double *y, double *jac);
 void ora_problem_fd_jacob(const ora_problem *p, double t, double pr, const double *cy, double *jac);

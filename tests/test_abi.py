@@ -93,3 +93,9 @@ def test_factor_test_rejects_bad_shapes_before_any_device_call():
         acc.factor_test(np.zeros((1, 3, 4)))
     with pytest.raises(ValueError):
         acc.factor_test(np.zeros((2, 14, 14)), b=np.zeros((2, 13)))
+
+
+def test_fd_order_outside_the_reference_set_is_rejected():
+    for order in (0, 3, 5, 8):
+        with pytest.raises(acc.AccelerIntError):
+            acc.set_fd_order(order)

@@ -467,6 +467,37 @@ def test_fd_jacobian_parity(session, oracle_lib, net14, system, precision):
         arbitrate("%s FD mixed" % system, y, yo, oracle_lib, om, y0, pr, dt, steps)
 
 
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("system", ["vdp", "k14"])
+def test_fd_jacobian_orders_parity(session, oracle_lib, net14, system, order):
+    """FD_ORD 2, 4 and 6 (central differences, generic/fd_jacob.c:33-66,103-119)
+    on the device against the oracle integrating with the same order, FP64."""
+    if system == "vdp":
+        num, steps, dt = 512, 10, 1e-3
+        y0, pr = M.van_der_pol_conditions(num, seed=81)
+        _vdp_setup(num)
+        om = oracle_lib.Mechanism.van_der_pol()
+    else:
+        num, steps, dt = 512, 3, 1e-6
+        y0, pr = M.initial_conditions(net14, num, seed=82)
+        _kin_setup(net14, num)
+        om = oracle_lib.Mechanism.kinetics(net14)
+    acc.set_jacobian(acc.JACOBIAN_FINITE_DIFFERENCE)
+    acc.set_fd_order(order)
+    try:
+        y = y0.copy()
+        acc.integrate(num, 0.0, steps * dt, dt, y, pr)
+        codes = acc.get_status(num)
+    finally:
+        acc.set_fd_order(1)
+        acc.set_jacobian(acc.JACOBIAN_ANALYTIC)
+    yo, co, _ = oracle_lib.integrate_steps(om, y0, pr, dt, steps, nthreads=8, fd_jacobian=True, fd_order=order)
+    assert (codes == co).all()
+    we = weighted_err(y, yo)
+    print("%s FD order %d: max weighted err vs oracle %.3g" % (system, order, we.max()))
+    assert we.max() <= 10.0
+
+
 def test_negative_stepsize_and_batches_beyond_capacity(session, oracle_lib):
     """solver_interface.cu conventions: stepsize < 0 means one global step of
     t_end - t_start (:164); NUM above the initialized capacity is integrated in

@@ -101,6 +101,28 @@ def test_reference_fd_jacobian_on_van_der_pol(oracle_lib):
     assert np.allclose(Jfd, J, rtol=1e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("order", [2, 4, 6])
+def test_fd_stencil_moments(oracle_lib, order):
+    """Central stencils of FD_ORD 2, 4, 6 (generic/fd_jacob.c:33-66): the
+    weights reproduce the derivative exactly on polynomials of degree <= order
+    (moment 1 equals 1, moments 0 and 2..order vanish) and not beyond."""
+    x, w = oracle_lib.fd_stencil(order)
+    assert len(x) == order and np.array_equal(x, -x[::-1]) and np.array_equal(w, -w[::-1])
+    for m in range(order + 1):
+        assert abs(np.sum(w * x ** m) - (1.0 if m == 1 else 0.0)) <= 1e-14, m
+    assert abs(np.sum(w * x ** (order + 1))) > 1e-3
+
+
+@pytest.mark.parametrize("order", [1, 2, 4, 6])
+def test_reference_fd_orders_on_van_der_pol(oracle_lib, order):
+    """Every FD_ORD of generic/fd_jacob.c against the analytic Jacobian."""
+    om = oracle_lib.Mechanism.van_der_pol()
+    y = np.array([1.3, -0.4])
+    J = oracle_lib.jacobian(om, 0.0, 50.0, y)
+    Jfd = oracle_lib.fd_jacobian(om, 0.0, 50.0, y, order=order)
+    assert np.allclose(Jfd, J, rtol=1e-5, atol=1e-6)
+
+
 def test_van_der_pol_jacobian(oracle_lib):
     om = oracle_lib.Mechanism.van_der_pol()
     y = np.array([1.3, -0.4])
