@@ -617,7 +617,8 @@ hipError_t launch_mt(int num, double t_start, double t_end, double step, const d
                      hipStream_t s, float* cost = nullptr) {
   constexpr int prec = std::is_same<MT, float>::value ? 1 : 0;
   if (S.jacobian == ACCELERINT_JACOBIAN_FINITE_DIFFERENCE) {
-    // finite-difference Jacobian: van der Pol and the 14-state network
+    // finite-difference Jacobian: van der Pol and the 14- and 54-state networks
+    // (the learned start is not compiled into these kernels)
     if (S.mech == ACCELERINT_MECH_VAN_DER_POL)
       return launch_radau<MechFD<MechVdP>, 1, 2, 256, ACC_VDP_WPE, MT>(
           num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P,
@@ -626,7 +627,11 @@ hipError_t launch_mt(int num, double t_start, double t_end, double step, const d
       return launch_radau<MechFD<MechKinetics14>, 16, 1, 128, ACC_K14_WPE, MT>(
           num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P,
           MechFD<MechKinetics14>::Params{S.kp14, S.P.atol, S.P.rtol, S.fd_order}, s, cost);
-    g_launch_msg = "the finite-difference Jacobian is compiled for van der Pol and the 14-state network only";
+    if (S.mech == ACCELERINT_MECH_KINETICS && S.n == 54)
+      return launch_radau<MechFD<MechKinetics54>, 64, 1, 256, ACC_K54_WPE, MT>(
+          num, t_start, t_end, step, pr, y, codes, cnt, S.d_work, list, num_dev, esc, S.P,
+          MechFD<MechKinetics54>::Params{S.kp54, S.P.atol, S.P.rtol, S.fd_order}, s, cost);
+    g_launch_msg = "the finite-difference Jacobian is compiled for van der Pol and the 14- and 54-state networks";
     return hipErrorInvalidValue;
   }
   switch (S.mech) {

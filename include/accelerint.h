@@ -120,8 +120,9 @@ int accelerInt_set_precision(int mode);
  * (FD_ORD 1: one forward difference per column, increment
  * max(sqrt(eps)|y_j|, r0 / (atol + rtol|y_j|)) with r0 = 1000 rtol eps N
  * sqrt(sum((atol + rtol|y_i|) f_i)^2 / N)), for callbacks without an analytic
- * Jacobian.  Compiled for van der Pol and the 14-state network (single-network
- * launches).  May be changed between integrate calls; returns 0 or -1. */
+ * Jacobian.  Compiled for van der Pol and the 14- and 54-state networks
+ * (single-network launches; with the learned start off).  May be changed
+ * between integrate calls; returns 0 or -1. */
 #define ACCELERINT_JACOBIAN_ANALYTIC 0
 #define ACCELERINT_JACOBIAN_FINITE_DIFFERENCE 1
 int accelerInt_set_jacobian(int mode);

@@ -422,8 +422,15 @@ def test_launch_shapes_agree_and_autotune(session, net14, tmp_path, system, prec
         acc.set_precision(acc.PRECISION_FP64)
 
 
-@pytest.mark.parametrize("system,precision", [("vdp", 0), ("k14", 0), ("k14", 1)])
-def test_fd_jacobian_parity(session, oracle_lib, net14, system, precision):
+# The 54-state network with FD takes FP64 only: there the reference's own FD
+# Jacobian leaves its results tens of tolerance units from the true solution
+# (its increment r0 / ewt is large for trace species), so a mixed-precision run
+# taking another step sequence cannot be held to the arbiter's bar of "no
+# farther from the truth than the reference" (measured: 255 of 256 conditions
+# within 10 x of the FP64 FD oracle, the last 10.4 from it and 57.9 from the
+# tight solution where the oracle is 47.6).
+@pytest.mark.parametrize("system,precision", [("vdp", 0), ("k14", 0), ("k14", 1), ("k54", 0)])
+def test_fd_jacobian_parity(session, oracle_lib, net14, net54, system, precision):
     """The finite-difference Jacobian (generic/fd_jacob.c) on the device against
     the oracle integrating with the same finite-difference Jacobian."""
     if system == "vdp":
@@ -431,6 +438,11 @@ def test_fd_jacobian_parity(session, oracle_lib, net14, system, precision):
         y0, pr = M.van_der_pol_conditions(num, seed=71)
         _vdp_setup(num)
         om = oracle_lib.Mechanism.van_der_pol()
+    elif system == "k54":
+        num, steps, dt = 256, 2, 1e-6
+        y0, pr = M.initial_conditions(net54, num, seed=73, T_range=(800.0, 2200.0))
+        _kin_setup(net54, num)
+        om = oracle_lib.Mechanism.kinetics(net54)
     else:
         num, steps, dt = 1024, 5, 1e-6
         y0, pr = M.initial_conditions(net14, num, seed=72)
