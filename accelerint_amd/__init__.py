@@ -211,7 +211,7 @@ def set_precision(mode: int):
 
 def set_jacobian(mode: int):
     """JACOBIAN_ANALYTIC (default) or JACOBIAN_FINITE_DIFFERENCE (generic/fd_jacob.c;
-    van der Pol and the 14-state network)."""
+    van der Pol and the 14- and 54-state networks)."""
     _check(lib().accelerInt_set_jacobian(int(mode)), "accelerInt_set_jacobian")
 
 
